@@ -1,0 +1,216 @@
+/*
+ * tisphi.h — C ABI of libtisphi_hip.so, the MI355X (gfx950) SPH particle engine.
+ *
+ * The reference (tiSPHi) has no FFI: its "operator API" is the Taichi
+ * @ti.data_oriented classes ParticleSystem / SPHBase / WCSPHSolver /
+ * MUISPHSolver / DPSPHSolver.  Each entry point below replaces one piece of
+ * that surface; the Python facade (tisphi_amd/eng/) binds them with ctypes
+ * and re-exposes the reference's class interface.
+ *
+ * Conventions
+ *   - Every function returns an int status: TSPH_OK (0) or a negative code.
+ *     tsph_last_error() returns a thread-local message for the last failure.
+ *     The library never aborts the process.
+ *   - An engine owns all device memory.  Host buffers are borrowed for the
+ *     duration of the call only.  Handles are opaque.
+ *   - One HIP stream per engine; an engine is not re-entrant, distinct
+ *     engines may be driven from distinct threads.
+ *   - An engine holds up to max_scenes independent scenes ("slots") of one
+ *     simulation method and dimension.  The single-scene functions
+ *     (tsph_create ...) build a one-slot engine and address slot 0.
+ *   - Host-side field data is float64 / int32 in the engine's current
+ *     particle order (the order produced by the last counting sort; right
+ *     after tsph_add_scene it is the upload order).  TSPH_F_ID0 carries the
+ *     generation index of each particle.
+ */
+#ifndef TISPHI_H
+#define TISPHI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSPH_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  TSPH_OK = 0,
+  TSPH_ERR_INVALID_ARG = -1,
+  TSPH_ERR_OOM = -2,
+  TSPH_ERR_HIP = -3,
+  TSPH_ERR_PARTICLE_ESCAPED = -4, /* a particle left the virtual grid      */
+  TSPH_ERR_NAN = -5,              /* NaN detected in density/velocity     */
+  TSPH_ERR_UNSUPPORTED = -6,      /* option of the scene format not built */
+  TSPH_ERR_CAPACITY = -7,         /* no free slot / particle / cell room  */
+};
+
+/* ---- particle material types (particle_system.py:23-24) ---------------- */
+enum {
+  TSPH_MAT_FLUID = 1,
+  TSPH_MAT_SOIL = 2,
+  TSPH_MAT_RIGID = 11,
+  TSPH_MAT_DUMMY = -1,
+  TSPH_MAT_REPULSIVE = -2,
+};
+
+/* ---- simulation methods (simulation.py:14-22) -------------------------- */
+enum { TSPH_METHOD_WC = 1, TSPH_METHOD_MUI = 2, TSPH_METHOD_DP = 3 };
+
+/* ---- scene description --------------------------------------------------
+ * Plain data derived from the scene JSON by the host
+ * (ParticleSystem.__init__, particle_system.py:14-58; SPHBase.__init__,
+ * solver_sph_base.py:14-21; solver constructors solver_sph_wc.py:12-21,
+ * solver_sph_muI.py:12-25, solver_sph_dp.py:12-29).                        */
+typedef struct tsph_scene_desc {
+  int32_t dim;              /* 2 or 3                                        */
+  int32_t method;           /* TSPH_METHOD_*                                 */
+  int32_t boundary;         /* 0 none, 1 collision, 2 dummy (3/4 rejected)   */
+  int32_t kernel;           /* 0 cubic spline, 1 Wendland C2                 */
+  int32_t kernel_corr;      /* 0 only                                        */
+  int32_t time_integration; /* 1 symplectic Euler, 2 leap-frog, 4 RK4        */
+  int32_t xsph;             /* XSPH position update on/off                   */
+  int32_t grid_num[3];      /* cells per axis; grid_num[2] == 1 in 2D        */
+  int64_t n_particles;      /* particles of this scene                      */
+  double vdomain_start[3];  /* virtual domain start (domain - grid)          */
+  double domain_start[3];   /* physical domain (collision boundary)         */
+  double domain_end[3];
+  double grid_size;         /* ceil(kappa*kh) * diameter                     */
+  double particle_radius;
+  double particle_diameter;
+  double smoothing_len;     /* kh * diameter                                 */
+  double support_radius;    /* kappa * smoothing_len                         */
+  double gravity[3];
+  double dt;                /* from calc_dt_CFL, solver_sph_base.py:209-212  */
+  /* material of mat_fluid[0] / mat_soil[0] */
+  double density0;
+  double viscosity;         /* WC                                            */
+  double stiffness;         /* WC Tait B                                     */
+  double exponent;          /* WC Tait gamma                                 */
+  double cohesion;          /* soil, Pa                                      */
+  double friction;          /* soil, radians                                 */
+  double young;             /* soil E                                        */
+  double poisson;           /* soil nu                                       */
+  double dilatancy;         /* soil, radians                                 */
+  double vsound;            /* solver sound speed                            */
+} tsph_scene_desc;
+
+/* ---- fields for tsph_upload / tsph_download -----------------------------
+ * count = n_particles of the slot; "x3" = 3 components per particle
+ * (row-major [n][3]); "x6" = stress components xx,yy,zz,xy,yz,zx.           */
+enum {
+  TSPH_F_POS = 0,           /* f64 x3, absolute position                    */
+  TSPH_F_VEL = 1,           /* f64 x3                                        */
+  TSPH_F_DENSITY = 2,       /* f64                                           */
+  TSPH_F_MASS = 3,          /* f64                                           */
+  TSPH_F_M_V = 4,           /* f64                                           */
+  TSPH_F_PRESSURE = 5,      /* f64                                           */
+  TSPH_F_TYPE = 6,          /* i32 mat_type                                  */
+  TSPH_F_ID0 = 7,           /* i32                                           */
+  TSPH_F_OBJ_ID = 8,        /* i32                                           */
+  TSPH_F_STRESS = 9,        /* f64 x6                                        */
+  TSPH_F_STRESS_TMP = 10,   /* f64 x6                                        */
+  TSPH_F_D_STRESS = 11,     /* f64 x6                                        */
+  TSPH_F_STRAIN_EQU = 12,   /* f64                                           */
+  TSPH_F_STRAIN_EQU_P = 13, /* f64                                           */
+  TSPH_F_D_STRAIN_EQU = 14, /* f64                                           */
+  TSPH_F_D_STRAIN_EQU_P = 15, /* f64                                         */
+  TSPH_F_FLAG_RETMAP = 16,  /* i32                                           */
+  TSPH_F_CSPM_F = 17,       /* f64 Shepard factor                            */
+  TSPH_F_D_DENSITY = 18,    /* f64                                           */
+  TSPH_F_D_VEL = 19,        /* f64 x3                                        */
+  TSPH_F_DENSITY_TMP = 20,  /* f64                                           */
+  TSPH_F_V_TMP = 21,        /* f64 x3                                        */
+  TSPH_F_CELL = 22,         /* i32 scene-local flattened cell id             */
+  TSPH_F_OFFSET = 23,       /* f64 x3 cell-local offset (stored as f32)      */
+  TSPH_F_CELL_START = 24,   /* i32 x (cells+1), download only: scene-local
+                               particle range of each cell                   */
+  TSPH_F_COUNT = 25
+};
+
+/* ---- single passes (test hooks; tsph_step runs the fused sequence) ----- */
+enum {
+  TSPH_PASS_SORT = 0,             /* hash + scan + stable counting sort     */
+  TSPH_PASS_SHEPARD = 1,          /* calc_CSPM_f                            */
+  TSPH_PASS_INIT_TMP = 2,         /* init_real2tmp                          */
+  TSPH_PASS_ONE_STEP = 3,         /* the method's one_step                  */
+  TSPH_PASS_LF_HALF = 4,          /* advect_LF_half                         */
+  TSPH_PASS_LF_FULL = 5,          /* advect_LF                              */
+  TSPH_PASS_ADVECT_POS = 6,       /* advect_pos (+XSPH)                     */
+  TSPH_PASS_ADVECT_SOMETHING = 7, /* solver-specific post-update            */
+  TSPH_PASS_STEP = 8,             /* one full step() via the pass sequence  */
+  TSPH_PASS_SE = 9,               /* advect_SE                              */
+  TSPH_PASS_ENFORCE_BOUNDARY = 10 /* enforce_boundary (boundary == 1)       */
+};
+
+/* ---- version / errors --------------------------------------------------- */
+int tsph_abi_version(void);
+const char* tsph_last_error(void);
+/* Number of HIP devices visible (0 without a GPU). */
+int tsph_device_count(int* n);
+
+/* ---- engine lifecycle --------------------------------------------------- */
+/* Engine for scenes of one method/dim.  Capacities bound the sum over
+ * scenes of particles and of grid cells. */
+int tsph_engine_create(int method, int dim, int64_t particle_capacity,
+                       int64_t cell_capacity, int max_scenes,
+                       void** out_engine);
+int tsph_engine_destroy(void* engine);
+/* Device bytes held by the engine (for HBM-budget admission). */
+int tsph_engine_bytes(void* engine, int64_t* out_bytes);
+
+/* ---- scenes ----------------------------------------------------------- */
+/* Reserve a slot and its particle/cell ranges; fields start at zero. */
+int tsph_add_scene(void* engine, const tsph_scene_desc* desc, int* out_slot);
+int tsph_remove_scene(void* engine, int slot);
+int tsph_upload(void* engine, int slot, int field, const void* host,
+                size_t bytes);
+int tsph_download(void* engine, int slot, int field, void* host,
+                  size_t bytes);
+
+/* ---- stepping ----------------------------------------------------------- */
+/* Advance every active scene by nsteps reference steps (step(),
+ * solver_sph_base.py:41-51).  Asynchronous on the engine stream; a scene
+ * leaves the stepping set with tsph_remove_scene. */
+int tsph_step(void* engine, int nsteps);
+/* Wait for the engine stream; reports escaped / NaN particles. */
+int tsph_sync(void* engine);
+/* Run one pass on every active scene (single-pass parity hooks). */
+int tsph_run_pass(void* engine, int pass_id);
+/* Steps taken by a slot since it was added. */
+int tsph_scene_steps(void* engine, int slot, int64_t* out_steps);
+/* Use a captured hipGraph for tsph_step (1, default) or plain launches (0). */
+int tsph_set_graph(void* engine, int enable);
+/* Threads per block for the pair kernels (64, 128, 256, 512, 1024). */
+int tsph_set_block(void* engine, int pair_block, int elem_block);
+
+/* ---- reductions --------------------------------------------------------- */
+/* FP64 global quantities of one scene:
+ * out[0] surge front = max x over flow particles,
+ * out[1] potential energy = sum m * (-g . x),
+ * out[2] kinetic energy   = sum 1/2 m |v|^2,
+ * out[3] mass             = sum m   (flow particles). */
+int tsph_globals(void* engine, int slot, double out[4]);
+
+/* ---- status / timing ---------------------------------------------------- */
+/* Device status word: out[0] escaped particles, out[1] NaN particles
+ * (counted since the scene was added). */
+int tsph_status(void* engine, int slot, int64_t out[2]);
+/* Average device time (ms) of the named kernel class over the launches
+ * since the last reset, measured with HIP events on the engine stream.
+ * name: "pair_fluid" etc.; returns count in *out_launches. */
+int tsph_kernel_time(void* engine, const char* name, double* out_ms,
+                     int64_t* out_launches);
+int tsph_timing(void* engine, int enable);
+
+/* ---- single-scene convenience (slot 0 of a one-slot engine) ------------ */
+int tsph_create(const tsph_scene_desc* desc, void** out_engine);
+int tsph_destroy(void* engine);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TISPHI_H */

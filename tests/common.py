@@ -1,0 +1,52 @@
+"""Shared helpers of the test suite (scene builders, tolerances)."""
+import copy
+import json
+import os
+
+import numpy as np
+
+from tisphi_amd.eng.configer_builder import SimConfiger
+from tisphi_amd.scene import build_scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SCENES = os.path.join(ROOT, "tisphi_amd", "scenes")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def scene_cfg(name):
+    return SimConfiger(os.path.join(SCENES, name + ".json"))
+
+
+def tiny_cfg(method=1, dim=2, xsph=False, r=None, domain=(0.2, 0.2, 0.06), block=(0.08, 0.08, 0.04),
+             boundary=2, ti=2, kernel=1):
+    """A small dam-break / column scene for brute-force checks."""
+    base = json.load(open(os.path.join(SCENES, "dam_break_2d.json" if method == 1 else
+                                       "column_collapse_mui.json")))
+    c = base["Configuration"]
+    c.update({"domainEnd": list(domain), "particleRadius": r or (0.005 if dim == 2 else 0.01),
+              "is2D": dim == 2, "simulationMethod": method, "xsph": xsph, "boundary": boundary,
+              "timeIntegration": ti, "kernel": kernel})
+    base["Blocks"][0]["size"] = list(block)
+    return SimConfiger(config=base)
+
+
+# Per-element parity tolerance |a - b| <= tol * max(|b|, floor) with the
+# floors of the measurement plan (rho0 for densities, rho0*|g|*H for
+# pressure and stress, |g| for accelerations, max|ref| for rates).
+TOL = 1e-5
+
+
+def floors(scene, H):
+    rho0 = scene.material["density0"]
+    g = float(np.linalg.norm(scene.gravity))
+    return dict(density=rho0, density_tmp=rho0, pressure=rho0 * g * H, stress=rho0 * g * H,
+                stress_tmp=rho0 * g * H, d_vel=g)
+
+
+def parity_err(a, b, floor):
+    """max |a-b| / max(|b|, floor)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if b.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))

@@ -1,0 +1,156 @@
+"""Scene format, geometry and particle generation (A1, A2, A16).
+
+Pinned by: the particle count in the reference's published 2D dam-break
+screenshot (img/sim_2022_12_14_db_WC_p.png: 9,144 particles, dt 1e-4), the
+closed forms of the lattice (np.arange from t + d/2 in steps of d, 'ij'
+order), the wall boxes of calc_dummy_boundary (particle_func.py:316-343)
+and the CFL rule (solver_sph_base.py:209-212).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from tests.common import GOLDEN, scene_cfg, tiny_cfg
+from tisphi_amd.eng.configer_builder import SimConfiger
+from tisphi_amd.scene import build_scene, calc_dt_cfl, derive_geometry
+
+
+def test_published_2d_dam_break_count():
+    sc = build_scene(scene_cfg("dam_break_2d"))
+    assert sc.n == 9144
+    assert sc.block_counts == [7200]
+    assert sc.n_dummy == 1944
+    assert sc.dt == pytest.approx(1e-4, rel=0, abs=1e-18)
+    assert list(sc.geo.grid_num) == [110, 56, 1]
+    assert sc.geo.grid_num_total == 6160
+
+
+def test_3d_dam_break_counts():
+    sc = build_scene(scene_cfg("dam_break_3d"), generate=False)
+    assert sc.block_counts == [2_880_000]
+    assert sc.n_dummy == 1_646_700
+    assert list(sc.geo.grid_num) == [217, 109, 36]
+    assert sc.geo.grid_num_total == 851_508
+    assert sc.dt == pytest.approx(1e-4, abs=1e-18)
+
+
+@pytest.mark.parametrize("name", ["column_collapse_mui", "column_collapse_dp"])
+def test_granular_column_counts(name):
+    sc = build_scene(scene_cfg(name))
+    assert sc.block_counts == [80_000]
+    assert sc.n_dummy == 4_818
+    assert list(sc.geo.grid_num) == [336, 102, 1]
+    assert sc.dt == pytest.approx(1e-5, abs=1e-18)
+
+
+def test_cfl_rule():
+    # c_dt = 0.2 h / c floors to zero under c_dt - c_dt % dt_min when it is
+    # below dt_min, so dt_min wins; otherwise it is truncated to a multiple.
+    assert calc_dt_cfl(0.2, 0.015, 60.0, 1e-4) == 1e-4
+    assert calc_dt_cfl(0.2, 0.0015, 53.5, 1e-5) == 1e-5
+    dt = calc_dt_cfl(0.2, 0.01, 1.0, 1e-4)  # 2e-3 is a multiple of 1e-4
+    assert dt == pytest.approx(2e-3, rel=1e-9)
+    assert calc_dt_cfl(0.2, 0.0125, 1.0, 1e-3) == pytest.approx(2e-3, rel=1e-9)
+
+
+def test_lattice_closed_form_and_order():
+    sc = build_scene(scene_cfg("dam_break_2d"))
+    d = sc.geo.particle_diameter
+    fl = sc.pos[:7200]
+    nx, ny = 120, 60
+    ix = np.arange(7200) // ny
+    iy = np.arange(7200) % ny
+    np.testing.assert_allclose(fl[:, 0], d / 2 + ix * d, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(fl[:, 1], d / 2 + iy * d, rtol=0, atol=1e-12)
+    assert np.all(fl[:, 2] == 0.0)
+    # dummy walls: left (x-), bottom (y-), right (x+), 3 layers each
+    du = sc.pos[7200:]
+    g = sc.geo.grid_size
+    left, bottom, right = du[:480], du[480:480 + 984], du[480 + 984:]
+    assert np.all(left[:, 0] < 0) and np.all(left[:, 1] > 0)
+    assert np.all(bottom[:, 1] < 0)
+    assert np.all(right[:, 0] > 3.22) and np.all(right[:, 1] > 0)
+    assert left[0, 0] == pytest.approx(-g + d / 2, abs=1e-12)
+    assert bottom[0, 0] == pytest.approx(-g + d / 2, abs=1e-12)
+    assert sc.density[7200:].max() == 0 and sc.mass[7200:].max() == 0
+    assert np.all(sc.mat_type[7200:] == -1) and np.all(sc.obj_id[7200:] == -1)
+    np.testing.assert_allclose(sc.m_V, d ** 2)
+    np.testing.assert_allclose(sc.mass[:7200], 1000 * d ** 2)
+
+
+def test_3d_wall_order():
+    sc = build_scene(tiny_cfg(method=1, dim=3))
+    geo = sc.geo
+    du = sc.pos[sc.block_counts[0]:]
+    ds, de = geo.domain_start, geo.domain_end
+    # x-, z+, x+, z-, y- : check the first particle of every wall box
+    walls = [w for w in __import__("tisphi_amd.eng.particle_func", fromlist=["x"]).calc_dummy_boundary(
+        3, ds, de, geo.vdomain_start, geo.vdomain_end)]
+    k = 0
+    for lo, hi in walls:
+        np.testing.assert_allclose(du[k], lo + geo.particle_diameter / 2, atol=1e-12)
+        n = 1
+        for a in range(3):
+            n *= len(np.arange(lo[a] + geo.particle_diameter / 2, hi[a] + 1e-5, geo.particle_diameter))
+        k += n
+    assert k == len(du)
+    assert np.all(du[-1][1] < ds[1])  # the floor comes last
+
+
+def test_geometry_2d_z_extent():
+    geo = derive_geometry(scene_cfg("dam_break_2d"))
+    d = geo.particle_diameter
+    assert geo.domain_end[2] == pytest.approx(geo.domain_start[2] + d)
+    assert geo.vdomain_start[2] == geo.domain_start[2]
+    assert geo.grid_num[2] == 1
+    assert geo.grid_size == pytest.approx(3 * d)
+    assert geo.support_radius == pytest.approx(2.0 * 1.5 * d)
+
+
+def test_scene_json_sections_and_errors(tmp_path):
+    cfg = json.load(open(os.path.join(GOLDEN, "scene_format.json")))
+    p = tmp_path / "s.json"
+    p.write_text(json.dumps(cfg))
+    c = SimConfiger(str(p))
+    assert c.get_cfg("kappa") == 2.0
+    assert c.get_materials()[0]["matType"] == 1
+    assert c.get_bodies() == [] and c.get_motions() != []
+    with pytest.raises(AssertionError):
+        c.get_cfg("noSuchKey", enforce_exist=True)
+    with pytest.raises(KeyError):
+        c.get_cfg("noSuchKey")
+    sc = build_scene(c)
+    # materialId indexes the Materials list position (the second entry)
+    assert np.all(sc.mat_type[:sc.block_counts[0]] == 1)
+    assert sc.density[0] == 998.0
+    cfg2 = json.loads(json.dumps(cfg))
+    cfg2["Configuration"]["simulationMethod"] = 7
+    with pytest.raises(NotImplementedError):
+        build_scene(SimConfiger(config=cfg2))
+    cfg3 = json.loads(json.dumps(cfg))
+    cfg3["Configuration"]["boundary"] = 3
+    with pytest.raises(NotImplementedError):
+        build_scene(SimConfiger(config=cfg3))
+    cfg4 = json.loads(json.dumps(cfg))
+    cfg4["Blocks"][0]["size"] = [9.0, 0.1, 0.1]
+    with pytest.raises(AssertionError):
+        build_scene(SimConfiger(config=cfg4))
+
+
+def test_dp_initial_geostatic_stress():
+    sc = build_scene(scene_cfg("column_collapse_dp"))
+    soil = sc.mat_type == 2
+    y = sc.pos[soil, 1]
+    k0 = 1 - math.sin(21.9 / 180 * math.pi)
+    syy = 2040 * -9.81 * (y.max() - y)
+    np.testing.assert_allclose(sc.stress[soil, 1], syy, rtol=1e-14, atol=1e-9)
+    np.testing.assert_allclose(sc.stress[soil, 0], k0 * syy, rtol=1e-14, atol=1e-9)
+    np.testing.assert_allclose(sc.stress[soil, 2], k0 * syy, rtol=1e-14, atol=1e-9)
+    assert np.all(sc.stress[:, 3:] == 0)
+    assert np.all(sc.stress[~soil] == 0)
+    # muI starts stress-free (init_stress is not called by its solver)
+    mui = build_scene(scene_cfg("column_collapse_mui"))
+    assert np.all(mui.stress == 0)

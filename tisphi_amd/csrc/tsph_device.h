@@ -1,0 +1,232 @@
+// tsph_device.h — device-side data layout and per-pair arithmetic of the
+// MI355X SPH engine (gfx950, wave64).
+//
+// Data layout in HBM (structure of arrays, one entry per particle slot of
+// the engine's particle capacity; every scene owns a contiguous range):
+//   cell   int32   scene-local flattened cell id  (ix*gny + iy)*gnz + iz
+//   pos    float4  cell-local offset x,y,z (metres) and m_V
+//   vel    float4  velocity and density
+//   vtmp   float4  v_tmp and density_tmp (leap-frog half state)
+//   mass   float, type int8, id0/obj int32, stress (6 comps, component-major)
+// Positions are an integer cell plus an FP32 offset from the cell corner:
+// the pair vector x_i - x_j = (c_i - c_j)*grid + (o_i - o_j) keeps ~1e-9 m
+// resolution anywhere in the domain.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsph {
+
+constexpr int kFluid = 1, kSoil = 2, kRigid = 11, kDummy = -1, kRep = -2;
+constexpr int kWC = 1, kMUI = 2, kDP = 3;
+constexpr int kWave = 64;
+
+__device__ __forceinline__ bool is_flow(int t) { return t == kFluid || t == kSoil; }
+__device__ __forceinline__ bool is_real(int t) { return t > 0; }
+__device__ __forceinline__ bool is_bdy(int t) { return t == kDummy || t == kRep; }
+
+// Stress components xx,yy,zz,xy,yz,zx.
+__device__ __forceinline__ constexpr int sidx(int a, int b) {
+  return a == b ? a : ((a + b) == 1 ? 3 : ((a + b) == 3 ? 4 : 5));
+}
+
+// Per-scene parameters (host fills from tsph_scene_desc; float copies are
+// the float64 values rounded once, like the reference's compile-time
+// constants folded into float32 code).
+struct SceneDev {
+  int32_t active, dim, method, boundary, kernel, ti, xsph, iexp;
+  int32_t gn[3];
+  int32_t tile_first, n_tiles;
+  int32_t pbase, n, cbase, ncell;
+  double grid, vds[3], dstart[3], dend[3], radius, dt_d;
+  float g, rs2, h1, kW, kC, eps;
+  float dt, half_dt, dt6, rho0, visc_c1, visc_c2, grav[3];
+  float stiffness, exponent, vsound2, coh, mu, young, h2;
+  double alpha, kc, G, K, sin_psi, eps_f;
+};
+
+// Particle state that survives a step (double-buffered; the counting sort
+// moves it from one buffer to the other).
+struct PState {
+  int32_t* cell;
+  float4* pos;
+  float4* vel;
+  float4* vtmp;
+  float* mass;
+  int8_t* type;
+  int32_t* id0;
+  int32_t* obj;
+  float* sig_f;    // 6 x cap, muI stress
+  double* sig_d;   // 6 x cap, DP stress
+  double* seq;     // strain_equ
+  double* seqp;    // strain_equ_p
+  int32_t* flag;   // flag_retmap
+};
+
+// Per-step work arrays (single-buffered).
+struct PWork {
+  int32_t* key;      // global cell key (pre-sort order)
+  int32_t* slot;     // arrival slot inside the cell (histogram atomics)
+  int32_t* tmp;      // scan position -> particle
+  int32_t* sid;      // particle -> scene slot (static per scene range)
+  float* p;          // pressure
+  float* cspm;       // Shepard factor
+  float* drho;       // d_density
+  float4* dv;        // d_vel
+  float* sigt_f;     // 6 x cap, stress_tmp in float (muI; DP mirror)
+  double* sigt_d;    // 6 x cap, DP stress_tmp
+  double* dsig_d;    // 6 x cap, DP d_stress
+  double* dseq;      // d_strain_equ
+  double* dseqp;     // d_strain_equ_p
+  float4* xs;        // XSPH sums
+  float* drho_rk;    // RK4 accumulators
+  float4* dv_rk;
+  double* dsig_rk;
+  int32_t* count;    // cell histogram (cell capacity)
+  int32_t* start;    // exclusive scan (cell capacity + 1)
+  int32_t* bsum;     // scan block sums
+  unsigned long long* status;  // per slot: escaped, NaN
+  double* gpart;     // globals partials per tile (4 each)
+  double* gout;      // globals per slot (4 each)
+};
+
+struct KArgs {
+  const SceneDev* __restrict__ scenes;
+  const int2* __restrict__ tiles;  // (slot, first particle) per 64-particle tile
+  PState cur, alt;
+  PWork w;
+  int32_t cap;     // particle capacity
+  int32_t ccap;    // cell capacity
+  int32_t stride;  // component stride of 6-vectors (== cap)
+};
+
+// ------------------------------------------------------------- kernels W
+// Wendland C2 (solver_sph_base.py:334-358) and cubic spline (:298-330).
+__device__ __forceinline__ float kernel_W(const SceneDev& S, float rn) {
+  float q = rn * S.h1;
+  if (!(rn > S.eps && q <= 2.0f)) return 0.0f;
+  if (S.kernel == 1) {
+    float q1 = 1.0f - 0.5f * q;
+    float q2 = q1 * q1;
+    return S.kW * (q2 * q2) * (1.0f + 2.0f * q);
+  }
+  if (q <= 1.0f) {
+    float qq = q * q;
+    return S.kC * (0.5f * (qq * q) - qq + (float)(2.0 / 3.0));
+  }
+  float t = 2.0f - q;
+  return S.kC / 6.0f * (t * t * t);
+}
+// grad W = fac * x_ij
+__device__ __forceinline__ float kernel_gradfac(const SceneDev& S, float rn) {
+  float q = rn * S.h1;
+  if (!(rn > S.eps && q <= 2.0f)) return 0.0f;
+  if (S.kernel == 1) {
+    float q1 = 1.0f - 0.5f * q;
+    float s = S.kW * ((q1 * q1) * q1) * (-5.0f * q) * S.h1;
+    return s / rn;
+  }
+  float coef;
+  if (q <= 1.0f) {
+    coef = S.kC * q * (1.5f * q - 2.0f);
+  } else {
+    float t = 2.0f - q;
+    coef = S.kC * (-0.5f * t * t);
+  }
+  return coef * S.h1 / rn;
+}
+
+template <class T>
+__device__ __forceinline__ T ipow_dev(T x, int n) {
+  T r = T(1), b = x;
+  int m = n < 0 ? -n : n;
+  while (m) {
+    if (m & 1) r = r * b;
+    b = b * b;
+    m >>= 1;
+  }
+  return n < 0 ? T(1) / r : r;
+}
+
+// Tait equation of state (solver_sph_wc.py:88).
+__device__ __forceinline__ float wc_eos(const SceneDev& S, float rt) {
+  float x = rt / S.rho0;
+  float xp = S.iexp != INT32_MIN ? ipow_dev(x, S.iexp) : powf(x, S.exponent);
+  float v = S.stiffness * (xp - 1.0f);
+  return v > 0.0f ? v : 0.0f;
+}
+
+// Cell of an offset along one axis: floor((double)o/grid) with the
+// offset renormalised into [0, grid) (same rule as the oracle).
+__device__ __forceinline__ void renorm(int32_t& c, float& o, double grid) {
+  double od = (double)o;
+  double k = floor(od / grid);
+  double r = od - k * grid;
+  if (r >= grid) {
+    k += 1.0;
+    r -= grid;
+  }
+  float on = (float)r;
+  if (on < 0.0f) on = 0.0f;
+  if ((double)on >= grid) {
+    k += 1.0;
+    on = 0.0f;
+  }
+  c += (int32_t)k;
+  o = on;
+}
+
+__device__ __forceinline__ void decode_cell(const SceneDev& S, int32_t c, int& x, int& y, int& z) {
+  int yz = S.gn[1] * S.gn[2];
+  x = c / yz;
+  int r = c - x * yz;
+  y = r / S.gn[2];
+  z = r - y * S.gn[2];
+}
+
+// The neighbour walk of for_all_neighbors (particle_system.py:259-269):
+// the 3^dim stencil in ndrange((-1,2),...) order around the centre cell
+// (scx,scy,scz), particles j ascending in each cell, j != i and
+// |x_ij|^2 < r_s^2.  Cells outside the grid are skipped.  (cix,ciy,ciz) is
+// the cell the offset `oi` is measured from (equal to the centre except
+// after advect_pos, before the next hash).  fn(j, x, y, z, r2).
+template <int DIM, class Fn>
+__device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restrict__ start,
+                                     const float4* __restrict__ pos, int32_t shift, int i,
+                                     int scx, int scy, int scz, int cix, int ciy, int ciz,
+                                     float oix, float oiy, float oiz, Fn&& fn) {
+  const int gy = S.gn[1], gz = S.gn[2];
+#pragma unroll 1
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int nx = scx + dx;
+    if (nx < 0 || nx >= S.gn[0]) continue;
+    const float bx = oix + (float)(cix - nx) * S.g;
+#pragma unroll 1
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int ny = scy + dy;
+      if (ny < 0 || ny >= gy) continue;
+      const float by = oiy + (float)(ciy - ny) * S.g;
+      const int32_t rowkey = S.cbase + (nx * gy + ny) * gz;
+#pragma unroll 1
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+        const int nz = scz + dz;
+        if (DIM == 3 && (nz < 0 || nz >= gz)) continue;
+        const float bz = DIM == 3 ? oiz + (float)(ciz - nz) * S.g : 0.0f;
+        const int32_t key = rowkey + nz;
+        const int32_t jb = start[key] + shift;
+        const int32_t je = start[key + 1] + shift;
+#pragma unroll 2
+        for (int32_t j = jb; j < je; ++j) {
+          const float4 pj = pos[j];
+          const float x = bx - pj.x;
+          const float y = by - pj.y;
+          const float z = DIM == 3 ? bz - pj.z : 0.0f;
+          const float r2 = x * x + y * y + z * z;
+          if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace tsph

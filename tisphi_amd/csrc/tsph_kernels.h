@@ -1,0 +1,1092 @@
+// tsph_kernels.h — HIP kernels of the SPH step (gfx950).
+//
+// Launch model: every particle kernel runs over a fixed grid of 64-particle
+// tiles; tiles[w] = (scene slot, first particle) for wave w, or slot -1 for
+// an unused tile.  A wave therefore always belongs to one scene, and the
+// scene parameters are read with wave-uniform (scalar) loads.  The grid is
+// sized from the engine capacity, so one captured hipGraph serves any set
+// of resident scenes.
+#pragma once
+#include "tsph_device.h"
+
+namespace tsph {
+
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+}
+
+// Thread -> (slot, particle).  False for idle lanes.
+__device__ __forceinline__ bool tile_thread(const KArgs& a, int& s, int& p) {
+  const int w = wave_id();
+  const int2 t = a.tiles[w];
+  if (t.x < 0) return false;
+  s = __builtin_amdgcn_readfirstlane(t.x);
+  p = t.y + (int)(threadIdx.x & 63);
+  const SceneDev& S = a.scenes[s];
+  return p < S.pbase + S.n;
+}
+
+__device__ __forceinline__ int32_t scene_shift(const KArgs& a, const SceneDev& S) {
+  return S.pbase - a.w.start[S.cbase];
+}
+
+// ===================================================================== sort
+// update_grid_id (particle_system.py:228-236): renormalise the offset,
+// flattened cell id, histogram with an arrival slot per particle.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_hash(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  int cx, cy, cz;
+  decode_cell(S, a.cur.cell[p], cx, cy, cz);
+  float4 P = a.cur.pos[p];
+  renorm(cx, P.x, S.grid);
+  renorm(cy, P.y, S.grid);
+  if (DIM == 3) {
+    renorm(cz, P.z, S.grid);
+  } else {
+    cz = 0;
+    P.z = 0.0f;
+  }
+  int esc = 0;
+  int* cc[3] = {&cx, &cy, &cz};
+  float* oo[3] = {&P.x, &P.y, &P.z};
+#pragma unroll
+  for (int ax = 0; ax < DIM; ++ax) {
+    int c = *cc[ax];
+    if (c < 0 || c >= S.gn[ax]) {
+      int cl = c < 0 ? 0 : S.gn[ax] - 1;
+      *oo[ax] = (float)((double)*oo[ax] + (double)(c - cl) * S.grid);
+      *cc[ax] = cl;
+      esc = 1;
+    }
+  }
+  if (esc) atomicAdd(&a.w.status[2 * s], 1ull);
+  const int32_t c = (cx * S.gn[1] + cy) * S.gn[2] + cz;
+  a.cur.cell[p] = c;
+  a.cur.pos[p] = P;
+  const int32_t key = S.cbase + c;
+  a.w.key[p] = key;
+  a.w.slot[p] = atomicAdd(&a.w.count[key], 1);
+}
+
+// Exclusive scan of the cell histogram (the inclusive prefix sum of
+// particle_system.py:256, stored shifted by one cell).  Three phases over
+// blocks of kScanBlock*kScanItems cells; the down-sweep also clears the
+// histogram for the next step.
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one int per thread; returns the total.
+__device__ __forceinline__ int block_excl_scan(int v, int& total) {
+  __shared__ int wsum[kScanBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kScanBlock / 64; ++k) {
+    if (k < wid) off += wsum[k];
+    tot += wsum[k];
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int32_t* __restrict__ count,
+                                                            int32_t* __restrict__ bsum, int n) {
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    if (i < n) s += count[i];
+  }
+  int total;
+  block_excl_scan(s, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_bsum(int32_t* __restrict__ bsum, int nb,
+                                                          int32_t* __restrict__ start, int n) {
+  // One block: exclusive scan of nb <= kScanTile block sums, in place.
+  int loc[kScanItems];
+  const int base = threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    loc[k] = i < nb ? bsum[i] : 0;
+    s += loc[k];
+  }
+  int total;
+  int off = block_excl_scan(s, total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    if (i < nb) bsum[i] = off;
+    off += loc[k];
+  }
+  if (threadIdx.x == 0) start[n] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_down(int32_t* __restrict__ count,
+                                                          const int32_t* __restrict__ bsum,
+                                                          int32_t* __restrict__ start, int n) {
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int loc[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    loc[k] = i < n ? count[i] : 0;
+    s += loc[k];
+  }
+  int total;
+  int off = block_excl_scan(s, total) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    if (i < n) {
+      start[i] = off;
+      count[i] = 0;
+    }
+    off += loc[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const int32_t key = a.w.key[p];
+  a.w.tmp[a.w.start[key] + a.w.slot[p]] = p;
+}
+
+// counting_sort (particle_system.py:238-252) with a deterministic slot:
+// rank among the particles of the same cell by previous index (the serial
+// reference order), then move the persistent state to the other buffer.
+__global__ __launch_bounds__(256) void k_rank_move(KArgs a, int move_vtmp, int move_mui,
+                                                   int move_dp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.w.start[a.ccap]) return;
+  const int p = a.w.tmp[k];
+  const int32_t key = a.w.key[p];
+  const int b = a.w.start[key], e = a.w.start[key + 1];
+  int rank = 0;
+  for (int q = b; q < e; ++q) rank += (a.w.tmp[q] < p) ? 1 : 0;
+  const int s = a.w.sid[p];
+  const SceneDev& S = a.scenes[s];
+  const int d = S.pbase + (b - a.w.start[S.cbase]) + rank;
+  a.alt.cell[d] = key - S.cbase;
+  a.alt.pos[d] = a.cur.pos[p];
+  a.alt.vel[d] = a.cur.vel[p];
+  a.alt.mass[d] = a.cur.mass[p];
+  a.alt.type[d] = a.cur.type[p];
+  a.alt.id0[d] = a.cur.id0[p];
+  a.alt.obj[d] = a.cur.obj[p];
+  if (move_vtmp) a.alt.vtmp[d] = a.cur.vtmp[p];
+  const int st = a.stride;
+  if (move_mui) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.alt.sig_f[c * st + d] = a.cur.sig_f[c * st + p];
+    a.alt.seq[d] = a.cur.seq[p];
+  }
+  if (move_dp) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.alt.sig_d[c * st + d] = a.cur.sig_d[c * st + p];
+    a.alt.seq[d] = a.cur.seq[p];
+    a.alt.seqp[d] = a.cur.seqp[p];
+    a.alt.flag[d] = a.cur.flag[p];
+  }
+}
+
+// ============================================================ pair helpers
+struct Centre {
+  int cx, cy, cz;
+  float4 P;
+  int32_t shift;
+};
+__device__ __forceinline__ Centre centre_of(const KArgs& a, const SceneDev& S, int p) {
+  Centre c;
+  decode_cell(S, a.cur.cell[p], c.cx, c.cy, c.cz);
+  c.P = a.cur.pos[p];
+  c.shift = scene_shift(a, S);
+  return c;
+}
+// Walk the stencil around particle p's own (renormalised) cell.
+template <int DIM, class Fn>
+__device__ __forceinline__ void walk_c(const SceneDev& S, const KArgs& a, int p, const Centre& C,
+                                       Fn&& fn) {
+  walk<DIM>(S, a.w.start, a.cur.pos, C.shift, p, C.cx, C.cy, C.cz, C.cx, C.cy, C.cz, C.P.x, C.P.y,
+            C.P.z, fn);
+}
+
+// =================================================================== Shepard
+// calc_CSPM_f (solver_sph_base.py:386-398)
+template <int DIM>
+__global__ __launch_bounds__(256) void k_shepard(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  float sum = 0.0f;
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    if (is_flow(type[j])) sum += mVj * kernel_W(S, sqrtf(r2));
+  });
+  a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
+}
+
+// init_real2tmp (solver_sph_base.py:67-74)
+__global__ __launch_bounds__(256) void k_init_tmp(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const int t = a.cur.type[p];
+  if (is_real(t)) a.cur.vtmp[p] = a.cur.vel[p];
+  if (t == kSoil) {
+    const int st = a.stride;
+    if (S.method == kMUI) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.w.sigt_f[c * st + p] = a.cur.sig_f[c * st + p];
+    } else if (S.method == kDP) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double v = a.cur.sig_d[c * st + p];
+        a.w.sigt_d[c * st + p] = v;
+        a.w.sigt_f[c * st + p] = (float)v;
+      }
+    }
+  }
+}
+
+// ===================================================================== WCSPH
+__global__ __launch_bounds__(256) void k_wc_eos(KArgs a) {  // solver_sph_wc.py:86-88
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] == kFluid) a.w.p[p] = wc_eos(a.scenes[s], a.cur.vtmp[p].w);
+}
+
+// Dummy extrapolation (solver_sph_wc.py:90-103; solver_sph_base.py:653-663).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (!is_bdy(a.cur.type[p])) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float* __restrict__ pr = a.w.p;
+  const float gy = S.grav[1];
+  float svx = 0.f, svy = 0.f, svz = 0.f, sp = 0.f;
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    if (!is_flow(type[j])) return;
+    const float w = kernel_W(S, sqrtf(r2));
+    const float4 vj = vtmp[j];
+    svx += mVj * vj.x * w;
+    svy += mVj * vj.y * w;
+    svz += mVj * vj.z * w;
+    sp += mVj * (pr[j] + vj.w * gy * y) * w;
+  });
+  const float cs = a.w.cspm[p];
+  const float4 v = a.cur.vel[p];
+  a.cur.vtmp[p] = make_float4(2.0f * v.x - svx * cs, 2.0f * v.y - svy * cs, 2.0f * v.z - svz * cs,
+                              S.rho0);
+  const float pv = sp * cs;
+  a.w.p[p] = pv > 0.0f ? pv : 0.0f;
+}
+
+// Fluid loop B (solver_sph_wc.py:108-123; tasks solver_sph_base.py:195-203,
+// solver_sph_wc.py:33-71): d_density and d_vel in one walk.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kFluid) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float* __restrict__ pr = a.w.p;
+  const float4 vi = vtmp[p];
+  const float pri = pr[p] / (vi.w * vi.w);
+  const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
+  float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f;
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    const float rn = sqrtf(r2);
+    const float f = kernel_gradfac(S, rn);
+    const float gx = f * x, gy = f * y, gz = f * z;
+    const float4 vj = vtmp[j];
+    const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
+    dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
+    const int tj = type[j];
+    float vis = 0.0f;
+    if (tj == kFluid || is_bdy(tj) || tj == kRigid) {
+      const float vx_ = ux * x + uy * y + uz * z;
+      const float mn = vx_ < 0.0f ? vx_ : 0.0f;
+      float aa = c1 * mVj;
+      if (tj != kFluid) aa = aa * rho0 / vi.w;
+      vis = aa * mn / (rn * rn + c2);
+    }
+    const float pc = (-rho0 * mVj) * (pri + pr[j] / (vj.w * vj.w));
+    dvx += vis * gx + pc * gx;
+    dvy += vis * gy + pc * gy;
+    dvz += vis * gz + pc * gz;
+  });
+  a.w.drho[p] = dd * vi.w;
+  a.w.dv[p] = make_float4(dvx + S.grav[0], dvy + S.grav[1], dvz + S.grav[2], 0.0f);
+}
+
+// ======================================================================= muI
+// Soil loop 1 (solver_sph_muI.py:200-225).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float4 vi = vtmp[p];
+  float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  float dd = 0.f;
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    const float f = kernel_gradfac(S, sqrtf(r2));
+    const float gw[3] = {f * x, f * y, f * z};
+    const float4 vj = vtmp[j];
+    const float u[3] = {mVj * (vj.x - vi.x), mVj * (vj.y - vi.y), mVj * (vj.z - vi.z)};
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) L[aa][bb] += u[aa] * gw[bb];
+    dd += (mVj * (vi.x - vj.x)) * gw[0] + (mVj * (vi.y - vj.y)) * gw[1] +
+          (mVj * (vi.z - vj.z)) * gw[2];
+  });
+  a.w.drho[p] = dd * vi.w;
+  float pp = S.vsound2 * (a.cur.vel[p].w - S.rho0);
+  pp = pp > 0.0f ? pp : 0.0f;
+  a.w.p[p] = pp;
+  float e[3][3];
+  float ss = 0.f;
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) e[aa][bb] = 0.5f * (L[aa][bb] + L[bb][aa]);
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) ss += e[aa][bb] * e[aa][bb];
+  const float en = sqrtf(0.5f * ss) + S.eps;
+  const float fac = 0.0f + (S.coh + pp * S.mu) / en;
+  const int st = a.stride;
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = aa; bb < 3; ++bb)
+      a.w.sigt_f[sidx(aa, bb) * st + p] = fac * e[aa][bb] - (aa == bb ? pp : 0.0f);
+  const float tr3 = (e[0][0] + e[1][1] + e[2][2]) / 3.0f;
+  float sd = 0.f;
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+      const float dlt = e[aa][bb] - (aa == bb ? tr3 : 0.0f);
+      sd += dlt * dlt;
+    }
+  a.w.dseq[p] = (double)sqrtf(sd * 2.0f / 3.0f);
+}
+
+// Dummy loop of muI (:228-242) and DP (:220-231).
+template <int DIM, bool EXTRAP_DENSITY>
+__global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (!is_bdy(a.cur.type[p])) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float* __restrict__ sgt = a.w.sigt_f;
+  const int st = a.stride;
+  float sv[3] = {0.f, 0.f, 0.f}, sr = 0.f, ssg[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    if (!is_flow(type[j])) return;
+    const float w = kernel_W(S, sqrtf(r2));
+    const float4 vj = vtmp[j];
+    sv[0] += mVj * vj.x * w;
+    sv[1] += mVj * vj.y * w;
+    sv[2] += mVj * vj.z * w;
+    sr += mVj * vj.w * w;
+    const float xr[3] = {x, y, z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ssg[c] += mVj * (sgt[c * st + j] + (vj.w * S.grav[c]) * xr[c]) * w;
+#pragma unroll
+    for (int c = 3; c < 6; ++c) ssg[c] += mVj * sgt[c * st + j] * w;
+  });
+  const float cs = a.w.cspm[p];
+  const float4 v = a.cur.vel[p];
+  float rt = S.rho0;
+  if (EXTRAP_DENSITY) {
+    const float r = sr * cs;
+    rt = r > S.rho0 ? r : S.rho0;
+  }
+  a.cur.vtmp[p] =
+      make_float4(2.0f * v.x - sv[0] * cs, 2.0f * v.y - sv[1] * cs, 2.0f * v.z - sv[2] * cs, rt);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const float val = ssg[c] * cs;
+    a.w.sigt_f[c * st + p] = val;
+    if (!EXTRAP_DENSITY) a.w.sigt_d[c * st + p] = (double)val;
+  }
+}
+
+// Acceleration from stress (muI :248-261; DP :260-270) + viscous damping
+// (solver_sph_base.py:713-715).
+template <int DIM>
+__device__ __forceinline__ void soil_dv_pair(const SceneDev& S, const float* __restrict__ sgt,
+                                             int st, int j, const float si[6], float mVj,
+                                             float rtj, float f, float x, float y, float z,
+                                             float dv[3]) {
+  const float gw[3] = {f * x, f * y, f * z};
+  const float rj2 = rtj * rtj;
+  const float sc = mVj * rtj;
+  float sj[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) sj[c] = sgt[c * st + j];
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa) {
+    float acc = 0.f;
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+      const int c = sidx(aa, bb);
+      acc += (sc * (sj[c] / rj2 + si[c])) * gw[bb];
+    }
+    dv[aa] += acc;
+  }
+}
+
+__device__ __forceinline__ float4 soil_dv_finish(const SceneDev& S, const float dv[3], float4 vi) {
+  const float damp = -5e-5f * sqrtf(S.young / (vi.w * S.h2));
+  return make_float4(dv[0] + S.grav[0] + damp * vi.x, dv[1] + S.grav[1] + damp * vi.y,
+                     dv[2] + S.grav[2] + damp * vi.z, 0.0f);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float* __restrict__ sgt = a.w.sigt_f;
+  const int st = a.stride;
+  const float4 vi = vtmp[p];
+  const float ri2 = vi.w * vi.w;
+  float si[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
+  float dv[3] = {0.f, 0.f, 0.f};
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    const float f = kernel_gradfac(S, sqrtf(r2));
+    soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vtmp[j].w, f, x, y, z, dv);
+  });
+  a.w.dv[p] = soil_dv_finish(S, dv, vi);
+}
+
+// advect_something_func, muI (:267-289), phase 1 of D2.
+__global__ __launch_bounds__(256) void k_mui_adv1(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const SceneDev& S = a.scenes[s];
+  float4 v = a.cur.vel[p];
+  v.w = v.w > S.rho0 ? v.w : S.rho0;
+  a.cur.vel[p] = v;
+  a.cur.pos[p].w = a.cur.mass[p] / v.w;
+  a.cur.seq[p] = (double)((float)a.cur.seq[p] + S.dt * (float)a.w.dseq[p]);
+  const int st = a.stride;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a.cur.sig_f[c * st + p] = a.w.sigt_f[c * st + p];
+}
+
+// Phase 2 of D2: stress regularisation from the post-advect positions; the
+// stencil centre is the cell of the new position, the pair vectors use the
+// stored (pre-advect) cells.  Result parked in sigt_f.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mui_regu(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  int sx = C.cx, sy = C.cy, sz = C.cz;
+  float tx = C.P.x, ty = C.P.y, tz = C.P.z;
+  renorm(sx, tx, S.grid);
+  renorm(sy, ty, S.grid);
+  if (DIM == 3) renorm(sz, tz, S.grid);
+  const int8_t* __restrict__ type = a.cur.type;
+  const float* __restrict__ sg = a.cur.sig_f;
+  const int st = a.stride;
+  const int ti = type[p];
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  walk<DIM>(S, a.w.start, a.cur.pos, C.shift, p, sx, sy, sz, C.cx, C.cy, C.cz, C.P.x, C.P.y,
+            C.P.z, [&](int j, float x, float y, float z, float r2, float mVj) {
+              if (type[j] != ti) return;
+              const float w = kernel_W(S, sqrtf(r2));
+#pragma unroll
+              for (int c = 0; c < 6; ++c) acc[c] += mVj * sg[c * st + j] * w;
+            });
+  const float cs = a.w.cspm[p];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a.w.sigt_f[c * st + p] = acc[c] * cs;
+}
+
+__global__ __launch_bounds__(256) void k_mui_adv2(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const int st = a.stride;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a.cur.sig_f[c * st + p] = a.w.sigt_f[c * st + p];
+}
+
+// ======================================================================== DP
+struct Inv {
+  double s[6], I1, sJ2, f;
+};
+__device__ __forceinline__ Inv dp_invariants(const SceneDev& S, const double st[6]) {
+  Inv v;
+  v.I1 = st[0] + st[1] + st[2];
+  const double m = v.I1 / 3.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v.s[c] = st[c] - (c < 3 ? m : 0.0);
+  double ss = 0.0;
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) ss += v.s[sidx(aa, bb)] * v.s[sidx(aa, bb)];
+  v.sJ2 = sqrt(0.5 * ss);
+  v.f = v.sJ2 + S.alpha * v.I1 - S.kc;
+  return v;
+}
+// adapt_stress (solver_sph_dp.py:73-96), FP64.
+__device__ __forceinline__ void dp_adapt(const SceneDev& S, double st[6]) {
+  Inv v = dp_invariants(S, st);
+  if (v.f > S.eps_f) {
+    if (v.f > v.sJ2) {
+      const double t = (v.I1 - S.kc / S.alpha) / 3.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) st[c] -= t;
+    }
+    v = dp_invariants(S, st);
+    const double r = (-v.I1 * S.alpha + S.kc) / v.sJ2;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) st[c] = r * v.s[c] + (c < 3 ? v.I1 / 3.0 : 0.0);
+  }
+}
+__device__ __forceinline__ int dp_flag(const SceneDev& S, const double st[6]) {
+  const Inv v = dp_invariants(S, st);
+  if (v.f < -S.eps_f) return 0;
+  if (v.f > S.eps_f) return v.f >= v.sJ2 ? 3 : 2;
+  return 1;
+}
+
+__global__ __launch_bounds__(256) void k_dp_adapt(KArgs a) {  // one_step loop 1
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const SceneDev& S = a.scenes[s];
+  const int st = a.stride;
+  double t[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) t[c] = a.w.sigt_d[c * st + p];
+  dp_adapt(S, t);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    a.w.sigt_d[c * st + p] = t[c];
+    a.w.sigt_f[c * st + p] = (float)t[c];
+  }
+}
+
+// calc_d_stress_Bui2008 (solver_sph_dp.py:171-208), FP64.
+__device__ void dp_bui2008(const SceneDev& S, const double L[3][3], const double st[6],
+                           double ds_out[6], double& dseq, double& dseqp) {
+  const Inv v = dp_invariants(S, st);
+  double e[3][3], w[3][3], Sg[3][3], s3[3][3];
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+      e[aa][bb] = 0.5 * (L[aa][bb] + L[bb][aa]);
+      w[aa][bb] = 0.5 * (L[aa][bb] - L[bb][aa]);
+      Sg[aa][bb] = st[sidx(aa, bb)];
+      s3[aa][bb] = v.s[sidx(aa, bb)];
+    }
+  const double tr = e[0][0] + e[1][1] + e[2][2];
+  double ed[3][3];
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) ed[aa][bb] = e[aa][bb] - (aa == bb ? tr / 3.0 : 0.0);
+  const bool plastic = v.f >= -S.eps_f && v.sJ2 > 1e-8;
+  double lam = 0.0;
+  if (plastic) {
+    double se = 0.0;
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) se += s3[aa][bb] * e[aa][bb];
+    lam = (3.0 * S.alpha * S.K * tr + (S.G / v.sJ2) * se) /
+          (27.0 * S.alpha * S.K * S.sin_psi + S.G);
+  }
+  double ds[3][3];
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+      const double J = Sg[aa][0] * w[bb][0] + Sg[aa][1] * w[bb][1] + Sg[aa][2] * w[bb][2] +
+                       Sg[0][bb] * w[aa][0] + Sg[1][bb] * w[aa][1] + Sg[2][bb] * w[aa][2];
+      const double gt =
+          plastic ? lam * ((aa == bb ? 9.0 * S.K * S.sin_psi : 0.0) + S.G / v.sJ2 * s3[aa][bb])
+                  : 0.0;
+      ds[aa][bb] = J + (2.0 * S.G * ed[aa][bb] + (aa == bb ? S.K * tr : 0.0)) - gt;
+    }
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = aa; bb < 3; ++bb) ds_out[sidx(aa, bb)] = ds[aa][bb];
+  double sd = 0.0;
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) sd += ed[aa][bb] * ed[aa][bb];
+  dseq = sqrt(sd * 2.0 / 3.0);
+  dseqp = 0.0;
+  if (plastic) {
+    const double gp = v.sJ2 + 3.0 * v.I1 * S.sin_psi;
+    double ep[3][3];
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+        ep[aa][bb] = (fabs(ds[aa][bb]) > 1e-8 ? gp / ds[aa][bb] : 0.0) * lam;
+    const double t3 = (ep[0][0] + ep[1][1] + ep[2][2]) / 3.0;
+    double q = 0.0;
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) {
+        const double d = ep[aa][bb] - (aa == bb ? t3 : 0.0);
+        q += d * d;
+      }
+    dseqp = sqrt(q * 2.0 / 3.0);
+  }
+}
+
+// DP soil loop (solver_sph_dp.py:237-270): velocity gradient, d_density
+// and the stress acceleration in one walk, then the FP64 stress rate.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float* __restrict__ sgt = a.w.sigt_f;
+  const int st = a.stride;
+  const float4 vi = vtmp[p];
+  const float ri2 = vi.w * vi.w;
+  float si[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
+  float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  float dd = 0.f, dv[3] = {0.f, 0.f, 0.f};
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    const float f = kernel_gradfac(S, sqrtf(r2));
+    const float gw[3] = {f * x, f * y, f * z};
+    const float4 vj = vtmp[j];
+    const float u[3] = {mVj * (vj.x - vi.x), mVj * (vj.y - vi.y), mVj * (vj.z - vi.z)};
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) L[aa][bb] += u[aa] * gw[bb];
+    dd += (mVj * (vi.x - vj.x)) * gw[0] + (mVj * (vi.y - vj.y)) * gw[1] +
+          (mVj * (vi.z - vj.z)) * gw[2];
+    soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vj.w, f, x, y, z, dv);
+  });
+  a.w.drho[p] = dd * vi.w;
+  double Ld[3][3], sd[6], ds[6], dq, dqp;
+#pragma unroll
+  for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) Ld[aa][bb] = (double)L[aa][bb];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) sd[c] = a.w.sigt_d[c * st + p];
+  dp_bui2008(S, Ld, sd, ds, dq, dqp);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a.w.dsig_d[c * st + p] = ds[c];
+  a.w.dseq[p] = dq;
+  a.w.dseqp[p] = dqp;
+  a.w.dv[p] = soil_dv_finish(S, dv, vi);
+}
+
+// advect_something_func, DP (:276-296).
+__global__ __launch_bounds__(256) void k_dp_adv(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kSoil) return;
+  const SceneDev& S = a.scenes[s];
+  float4 v = a.cur.vel[p];
+  v.w = v.w > S.rho0 ? v.w : S.rho0;
+  a.cur.vel[p] = v;
+  a.cur.pos[p].w = a.cur.mass[p] / v.w;
+  const int st = a.stride;
+  double t[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) t[c] = a.cur.sig_d[c * st + p];
+  a.cur.flag[p] = dp_flag(S, t);
+  dp_adapt(S, t);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a.cur.sig_d[c * st + p] = t[c];
+  a.cur.seq[p] += S.dt_d * a.w.dseq[p];
+  a.cur.seqp[p] += S.dt_d * a.w.dseqp[p];
+}
+
+// =============================================================== integrators
+// advect_LF_half (solver_sph_base.py:96-104).
+__global__ __launch_bounds__(256) void k_lf_half(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const int t = a.cur.type[p];
+  if (is_real(t)) {
+    float4 vt = a.cur.vtmp[p];
+    const float4 d = a.w.dv[p];
+    vt.w += S.half_dt * a.w.drho[p];
+    a.cur.pos[p].w = a.cur.mass[p] / vt.w;
+    vt.x += S.half_dt * d.x;
+    vt.y += S.half_dt * d.y;
+    vt.z += S.half_dt * d.z;
+    a.cur.vtmp[p] = vt;
+  }
+  if (t == kSoil && S.method == kDP) {
+    const int st = a.stride;
+    const double h = 0.5 * S.dt_d;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double v = a.w.sigt_d[c * st + p] + h * a.w.dsig_d[c * st + p];
+      a.w.sigt_d[c * st + p] = v;
+      a.w.sigt_f[c * st + p] = (float)v;
+    }
+  }
+}
+
+// advect_LF (:106-114); advect_SE (:79-87) is the same update.
+__global__ __launch_bounds__(256) void k_lf_full(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const int t = a.cur.type[p];
+  if (is_real(t)) {
+    float4 v = a.cur.vel[p];
+    const float4 d = a.w.dv[p];
+    v.w += S.dt * a.w.drho[p];
+    a.cur.pos[p].w = a.cur.mass[p] / v.w;
+    v.x += S.dt * d.x;
+    v.y += S.dt * d.y;
+    v.z += S.dt * d.z;
+    a.cur.vel[p] = v;
+    if (isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w)) atomicAdd(&a.w.status[2 * s + 1], 1ull);
+  }
+  if (t == kSoil && S.method == kDP) {
+    const int st = a.stride;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.cur.sig_d[c * st + p] += S.dt_d * a.w.dsig_d[c * st + p];
+  }
+}
+
+// XSPH sum (solver_sph_base.py:223-226), from pre-advect positions (D3).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_xsph(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const int ti = a.cur.type[p];
+  if (!is_real(ti)) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  const float4* __restrict__ vel = a.cur.vel;
+  const float4 vi = vel[p];
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    if (type[j] != ti) return;
+    const float w = kernel_W(S, sqrtf(r2));
+    const float4 vj = vel[j];
+    sx += mVj * (vj.x - vi.x) * w;
+    sy += mVj * (vj.y - vi.y) * w;
+    sz += mVj * (vj.z - vi.z) * w;
+  });
+  a.w.xs[p] = make_float4(sx, sy, sz, 0.f);
+}
+
+// advect_pos (solver_sph_base.py:228-238): offsets move, cells are
+// re-derived by the next hash.
+__global__ __launch_bounds__(256) void k_advect_pos(KArgs a, int use_xsph) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (!is_real(a.cur.type[p])) return;
+  const SceneDev& S = a.scenes[s];
+  const float4 v = a.cur.vel[p];
+  float4 P = a.cur.pos[p];
+  if (use_xsph && S.xsph) {
+    const float4 xs = a.w.xs[p];
+    P.x += S.dt * (v.x + 0.5f * xs.x);
+    P.y += S.dt * (v.y + 0.5f * xs.y);
+    P.z += S.dt * (v.z + 0.5f * xs.z);
+  } else {
+    P.x += S.dt * v.x;
+    P.y += S.dt * v.y;
+    P.z += S.dt * v.z;
+  }
+  if (S.dim == 2) P.z = 0.0f;
+  a.cur.pos[p] = P;
+}
+
+// chk_density for fluid (WC advect_something, solver_sph_wc.py:129-133).
+__global__ __launch_bounds__(256) void k_chk_density(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  if (a.cur.type[p] != kFluid) return;
+  const SceneDev& S = a.scenes[s];
+  float4 v = a.cur.vel[p];
+  v.w = v.w > S.rho0 ? v.w : S.rho0;
+  a.cur.vel[p] = v;
+  a.cur.pos[p].w = a.cur.mass[p] / v.w;
+}
+
+// enforce_boundary_2D/3D (solver_sph_base.py:540-601) for boundary == 1.
+__global__ __launch_bounds__(256) void k_enforce_boundary(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  if (S.boundary != 1 || !is_flow(a.cur.type[p])) return;
+  int cc[3];
+  decode_cell(S, a.cur.cell[p], cc[0], cc[1], cc[2]);
+  float4 P = a.cur.pos[p];
+  float* o[3] = {&P.x, &P.y, &P.z};
+  double pos[3];
+  for (int ax = 0; ax < 3; ++ax)
+    pos[ax] = (ax == 2 && S.dim == 2) ? 0.0 : S.vds[ax] + (double)cc[ax] * S.grid + (double)*o[ax];
+  const double dist = S.radius - 1e-8;
+  double nrm[3] = {0.0, 0.0, 0.0};
+  auto setp = [&](int ax, double x) { *o[ax] = (float)(x - S.vds[ax] - (double)cc[ax] * S.grid); };
+  if (pos[0] > S.dend[0] - dist) { nrm[0] += 1.0; setp(0, S.dend[0] - dist); }
+  if (pos[0] <= S.dstart[0] + dist) { nrm[0] -= 1.0; setp(0, S.dstart[0] + dist); }
+  if (pos[1] <= S.dstart[1] + dist) { nrm[1] -= 1.0; setp(1, S.dstart[1] + dist); }
+  if (S.dim == 3) {
+    if (pos[2] > S.dend[2] - dist) { nrm[2] += 1.0; setp(2, S.dend[2] - dist); }
+    if (pos[2] <= S.dstart[2] + dist) { nrm[2] -= 1.0; setp(2, S.dstart[2] + dist); }
+  }
+  a.cur.pos[p] = P;
+  const double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  if (len > 1e-8) {
+    const float u[3] = {(float)(nrm[0] / len), (float)(nrm[1] / len), (float)(nrm[2] / len)};
+    float4 v = a.cur.vel[p];
+    const float k = 1.3f * (v.x * u[0] + v.y * u[1] + v.z * u[2]);
+    v.x -= k * u[0];
+    v.y -= k * u[1];
+    v.z -= k * u[2];
+    a.cur.vel[p] = v;
+  }
+}
+
+// ======================================================================= RK4
+// substep_RK (solver_sph_base.py:134-180).
+__global__ __launch_bounds__(256) void k_rk_init(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const int t = a.cur.type[p];
+  if (is_real(t)) {
+    a.w.drho_rk[p] = 0.f;
+    a.w.dv_rk[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (t == kSoil) {
+    const int st = a.stride;
+    for (int c = 0; c < 6; ++c) a.w.dsig_rk[c * st + p] = 0.0;
+  }
+}
+__global__ __launch_bounds__(256) void k_rk_update(KArgs a, int m) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const int t = a.cur.type[p];
+  const float mf = (float)m;
+  if (is_real(t)) {
+    a.w.drho_rk[p] += a.w.drho[p] * mf;
+    float4 r = a.w.dv_rk[p];
+    const float4 d = a.w.dv[p];
+    r.x += d.x * mf;
+    r.y += d.y * mf;
+    r.z += d.z * mf;
+    a.w.dv_rk[p] = r;
+  }
+  if (t == kSoil && S.method == kDP) {
+    const int st = a.stride;
+    for (int c = 0; c < 6; ++c) a.w.dsig_rk[c * st + p] += a.w.dsig_d[c * st + p] * m;
+  }
+}
+__global__ __launch_bounds__(256) void k_rk_advect4(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const int t = a.cur.type[p];
+  if (is_real(t)) {
+    const float4 v = a.cur.vel[p];
+    const float4 d = a.w.dv[p];
+    float4 vt;
+    vt.w = S.half_dt * a.w.drho[p] + v.w;
+    a.cur.pos[p].w = a.cur.mass[p] / vt.w;
+    vt.x = S.half_dt * d.x + v.x;
+    vt.y = S.half_dt * d.y + v.y;
+    vt.z = S.half_dt * d.z + v.z;
+    a.cur.vtmp[p] = vt;
+  }
+  if (t == kSoil && S.method == kDP) {
+    const int st = a.stride;
+    for (int c = 0; c < 6; ++c) {
+      const double v = 0.5 * S.dt_d * a.w.dsig_d[c * st + p] + a.cur.sig_d[c * st + p];
+      a.w.sigt_d[c * st + p] = v;
+      a.w.sigt_f[c * st + p] = (float)v;
+    }
+  }
+}
+__global__ __launch_bounds__(256) void k_rk_advect(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const int t = a.cur.type[p];
+  if (is_real(t)) {
+    float4 v = a.cur.vel[p];
+    const float4 d = a.w.dv_rk[p];
+    v.w += S.dt6 * a.w.drho_rk[p];
+    a.cur.pos[p].w = a.cur.mass[p] / v.w;
+    v.x += S.dt6 * d.x;
+    v.y += S.dt6 * d.y;
+    v.z += S.dt6 * d.z;
+    a.cur.vel[p] = v;
+  }
+  if (t == kSoil && S.method == kDP) {
+    const int st = a.stride;
+    for (int c = 0; c < 6; ++c)
+      a.cur.sig_d[c * st + p] += S.dt_d / 6.0 * a.w.dsig_rk[c * st + p];
+  }
+}
+
+// =================================================================== globals
+// FP64 front / potential / kinetic / mass over flow particles.  Partials per
+// 64-particle tile with a fixed shuffle tree, then one block per scene sums
+// the scene's tiles in a fixed order: results do not depend on where the
+// scene sits in the engine or on what else is resident.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__global__ __launch_bounds__(256) void k_globals_tile(KArgs a) {
+  const int w = wave_id();
+  const int2 t = a.tiles[w];
+  if (t.x < 0) return;
+  const int s = t.x;
+  const SceneDev& S = a.scenes[s];
+  const int p = t.y + (int)(threadIdx.x & 63);
+  double fr = -INFINITY, pe = 0.0, ke = 0.0, m = 0.0;
+  if (p < S.pbase + S.n && is_flow(a.cur.type[p])) {
+    int cc[3];
+    decode_cell(S, a.cur.cell[p], cc[0], cc[1], cc[2]);
+    const float4 P = a.cur.pos[p];
+    const double x = S.vds[0] + (double)cc[0] * S.grid + (double)P.x;
+    const double y = S.vds[1] + (double)cc[1] * S.grid + (double)P.y;
+    const double z = S.dim == 3 ? S.vds[2] + (double)cc[2] * S.grid + (double)P.z : 0.0;
+    const float4 v = a.cur.vel[p];
+    const double mi = (double)a.cur.mass[p];
+    fr = x;
+    pe = mi * -((double)S.grav[0] * x + (double)S.grav[1] * y + (double)S.grav[2] * z);
+    ke = 0.5 * mi * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+    m = mi;
+  }
+  fr = wave_max(fr);
+  pe = wave_sum(pe);
+  ke = wave_sum(ke);
+  m = wave_sum(m);
+  if ((threadIdx.x & 63) == 0) {
+    double* o = a.w.gpart + 4 * (size_t)w;
+    o[0] = fr;
+    o[1] = pe;
+    o[2] = ke;
+    o[3] = m;
+  }
+}
+__global__ __launch_bounds__(256) void k_globals_scene(KArgs a, int nslots) {
+  const int s = blockIdx.x;
+  if (s >= nslots) return;
+  const SceneDev& S = a.scenes[s];
+  if (!S.active) return;
+  double fr = -INFINITY, pe = 0.0, ke = 0.0, m = 0.0;
+  for (int k = threadIdx.x; k < S.n_tiles; k += blockDim.x) {
+    const double* o = a.w.gpart + 4 * (size_t)(S.tile_first + k);
+    fr = fmax(fr, o[0]);
+    pe += o[1];
+    ke += o[2];
+    m += o[3];
+  }
+  __shared__ double red[4][256 / 64];
+  fr = wave_max(fr);
+  pe = wave_sum(pe);
+  ke = wave_sum(ke);
+  m = wave_sum(m);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = fr;
+    red[1][wid] = pe;
+    red[2][wid] = ke;
+    red[3][wid] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      fr = fmax(fr, red[0][k]);
+      pe += red[1][k];
+      ke += red[2][k];
+      m += red[3][k];
+    }
+    double* o = a.w.gout + 4 * s;
+    o[0] = fr;
+    o[1] = pe;
+    o[2] = ke;
+    o[3] = m;
+  }
+}
+
+}  // namespace tsph

@@ -1,0 +1,154 @@
+"""SPHBase with the reference's interface (eng/solver_sph_base.py:10-793).
+
+``step()`` (:41-51) launches the fused device sequence for one reference
+step; the individual reference kernels are exposed as single device passes
+for callers that drive them one by one.
+"""
+import math
+
+import numpy as np
+
+from .particle_system import ScalarField
+
+
+class SPHBase:
+    def __init__(self, particle_system):
+        self.ps = particle_system
+        cfg = self.ps.cfg
+        self.g = np.array(cfg.get_cfg("gravitation"), dtype=np.float64)
+        self.flagKernel = cfg.get_cfg("kernel")
+        self.flagKernelCorr = cfg.get_cfg("kernelCorrection")
+        self.flagTI = cfg.get_cfg("timeIntegration")
+        self.flagXSPH = cfg.get_cfg("xsph")
+        self.dt_min = cfg.get_cfg("timeStepSizeMin")
+        self.dt = ScalarField(lambda: self.ps.scene.dt)
+        self.I = np.eye(self.ps.dim)
+        self.I3 = np.eye(3)
+        self.epsilon = 1e-8
+        self.alert_ratio = 0.01
+
+    @staticmethod
+    def calc_dt_CFL(CFL_component, vsound, dt_min, smoothing_len):
+        dt = CFL_component * smoothing_len / vsound
+        return max(dt_min, dt - dt % dt_min)
+
+    # ------------------------------------------------------ time integration
+    def step(self):
+        self.ps.engine.step(1)
+
+    def steps(self, n):
+        """``n`` consecutive step() calls in one submission."""
+        self.ps.engine.step(int(n))
+
+    def sync(self):
+        self.ps.engine.sync()
+
+    def _pass(self, name):
+        self.ps.engine.run(name)
+
+    def substep(self):
+        if self.flagTI == 1:
+            self.substep_SE()
+        elif self.flagTI == 2:
+            self.substep_LF()
+        elif self.flagTI == 4:
+            self.substep_RK()
+        else:
+            raise NotImplementedError(f"timeIntegration {self.flagTI}")
+
+    def one_step(self):
+        self._pass("one_step")
+
+    def init_real2tmp(self):
+        self._pass("init_tmp")
+
+    def advect_SE(self):
+        self._pass("se")
+
+    def substep_SE(self):
+        self.one_step()
+        self.advect_SE()
+
+    def advect_LF_half(self):
+        self._pass("lf_half")
+
+    def advect_LF(self):
+        self._pass("lf_full")
+
+    def substep_LF(self):
+        self.one_step()
+        self.advect_LF_half()
+        self.one_step()
+        self.advect_LF()
+
+    def substep_RK(self):
+        raise NotImplementedError("RK4 runs inside step(); its stages are not exposed one by one")
+
+    def advect_pos(self):
+        self._pass("advect_pos")
+
+    def advect_something(self):
+        self._pass("advect_something")
+
+    def calc_kernel_corr(self):
+        self.calc_CSPM_f()
+
+    def calc_CSPM_f(self):
+        self._pass("shepard")
+
+    def enforce_boundary(self):
+        self._pass("enforce_boundary")
+
+    def solve_rigid_body(self):
+        """No rigid bodies are resident (rigid blocks are rejected at load)."""
+
+    # ----------------------------------------------------------- visuals
+    def assign_value_color(self):
+        """Colour value per particle (solver_sph_base.py:721-789), computed
+        on the host from downloaded fields; returns the array."""
+        ps, e, s = self.ps, self.ps.engine, self.ps.slot
+        t = ps.color_title
+        if t in (2,):
+            val = e.download(s, "density")
+        elif t == 1:
+            val = e.download(s, "id0").astype(np.float64)
+        elif t == 21:
+            val = e.download(s, "d_density")
+        elif t in (3, 31, 32, 33):
+            v = e.download(s, "vel")
+            val = {3: np.linalg.norm(v, axis=1), 31: v[:, 0], 32: -v[:, 1], 33: v[:, 2]}[t]
+        elif t in (34, 35, 36, 37):
+            v = e.download(s, "d_vel")
+            val = {34: np.linalg.norm(v, axis=1), 35: v[:, 0], 36: v[:, 1], 37: v[:, 2]}[t]
+        elif t in (4, 41, 42, 43, 44):
+            x = e.download(s, "pos")
+            if t == 44:
+                x0 = ps.scene.pos[e.download(s, "id0")]
+                val = np.linalg.norm(x - x0, axis=1)
+            else:
+                val = {4: np.linalg.norm(x, axis=1), 41: x[:, 0], 42: x[:, 1], 43: x[:, 2]}[t]
+        elif t in (51, 52, 53, 54, 55, 56, 57):
+            st = e.download(s, "stress")
+            val = {51: st[:, 0], 52: -st[:, 1], 53: -st[:, 2], 54: st[:, 3], 55: st[:, 4],
+                   56: st[:, 5], 57: -(st[:, 0] + st[:, 1] + st[:, 2]) / 3.0}[t]
+        elif t == 61:
+            val = e.download(s, "strain_equ")
+        elif t == 62:
+            val = e.download(s, "strain_equ_p")
+        elif t == 7:
+            val = e.download(s, "pressure")
+        elif t == 8:
+            val = e.download(s, "flag_retmap").astype(np.float64)
+        elif t == 100:
+            val = e.download(s, "cell").astype(np.float64)
+        elif t == 103:
+            val = np.linalg.norm(e.download(s, "v_tmp"), axis=1)
+        else:
+            val = np.zeros(ps.particle_max_num)
+        self._val = val
+        return val
+
+    # material helpers used by the concrete solvers
+    @staticmethod
+    def _deg2rad(x):
+        return x / 180 * math.pi
