@@ -1,0 +1,158 @@
+"""Benchmark: fluid particle-steps/s of the 3D WCSPH dam break on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 3D dam
+break, water block 1.2 x 0.6 x 0.5 m in a 3.22 x 1.6 x 0.5 m box, particle
+radius 0.0025 m (240 x 120 x 100 = 2,880,000 fluid particles) plus
+1,646,700 dummy-wall particles, rho0 1000, Tait B 5e4 / gamma 7,
+viscosity 0.005, Wendland C2 (kappa 2, kh 1.5), leap-frog, dt 1e-4, f32.
+Inputs are generated on the host and resident in HBM before timing starts.
+
+Prints one JSON line; see the keys below.  ``--gpus`` > 1 is refused: this
+engine runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 2.9M 3D WCSPH; batched/served scene p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+PAIR_KERNEL = "wc_fluid"
+
+
+def algorithmic_bytes_fluid_pass(dim, n_all, n_fluid):
+    """Bytes per launch of the fluid pair pass: every particle's record is
+    read once (cell 4, offset 4*dim, v_tmp 4*dim, rho_tmp 4, p 4, m_V 4,
+    type 1) and d_density + d_vel written once per fluid particle."""
+    rec = 41 if dim == 3 else 33
+    out = 16 if dim == 3 else 12
+    return rec * n_all + out * n_fluid
+
+
+def algorithmic_flops_fluid_pass(dim, n_fluid):
+    """FP32 flops per launch: candidates x 8 + neighbours x 51 per fluid
+    particle (27 cells x 27 / 9 cells x 9 candidates; 92 / 24 lattice
+    neighbours)."""
+    cand, nb = (729, 92) if dim == 3 else (81, 24)
+    return n_fluid * (cand * 8 + nb * 51)
+
+
+def read_traffic(path):
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(scene, steps):
+    """The FP64 oracle (C++/OpenMP restatement of the reference) timed on
+    the host cores on ``steps`` steps of the same scene."""
+    from oracle import oracle as orc
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    orc.set_threads(cores)
+    o = orc.Oracle(scene.desc(), 64)
+    orc.load_scene(o, scene)
+    o.step(1)  # warm-up (first sort of the generated order)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return dict(value=scene.n_flow * steps / dt, unit="particle-steps/s", cores=cores, kind="port",
+                sample=f"{steps} steps of the full workload after 1 warm-up step, FP64 oracle "
+                       f"({dt:.1f} s, {os.cpu_count()} logical CPUs visible)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scene", default="dam_break_3d")
+    ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: multi-process launch is not supported (one GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    from tisphi_amd import _lib as L
+    from tisphi_amd.eng.configer_builder import SimConfiger
+    from tisphi_amd.scene import build_scene
+
+    sc = build_scene(SimConfiger(os.path.join(ROOT, "tisphi_amd", "scenes", args.scene + ".json")))
+    dim = sc.geo.dim
+    eng = L.Engine(sc.method, dim, sc.n, sc.geo.grid_num_total, 1)
+    slot = eng.add_scene(sc.desc())
+    L.upload_scene(eng, slot, sc)
+    eng.set_graph(args.graph)
+    eng.step(args.warmup)
+    eng.sync()
+    if not args.graph:
+        eng.timing(True)  # HIP events around every launch on the engine stream
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.sync()
+    t = time.perf_counter() - t0
+    value = sc.n_flow * args.steps / t
+    ms_pair, n_pair = eng.kernel_time(PAIR_KERNEL) if not args.graph else (0.0, 0)
+    kernels = {}
+    if not args.graph:
+        for k in ("hash", "scan", "scatter", "rank_move", "shepard", "init_tmp", "wc_eos", "wc_bdy",
+                  "wc_fluid", "lf_half", "lf_full", "advect_pos", "chk_density"):
+            ms, n = eng.kernel_time(k)
+            if n:
+                kernels[k] = round(ms * n / args.steps, 5)
+    status = eng.status(slot)
+    g = eng.globals(slot)
+    eng.close()
+
+    alg = algorithmic_bytes_fluid_pass(dim, sc.n, sc.n_flow)
+    flops = algorithmic_flops_fluid_pass(dim, sc.n_flow)
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"wc_fluid_{args.scene}_pmc.txt"))
+    roof = None
+    if ms_pair > 0:
+        ach = alg / (ms_pair * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 5), traffic=traffic,
+                    kernel=f"k_wc_fluid<{dim}>", launches=n_pair, avg_launch_ms=round(ms_pair, 5),
+                    algorithmic_bytes_per_launch=alg,
+                    valu=dict(achieved_tflops=round(flops / (ms_pair * 1e-3) / 1e12, 3),
+                              peak_tflops=FP32_PEAK_TFLOPS,
+                              frac=round(flops / (ms_pair * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 5),
+                              flops_per_launch=flops))
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: lattice dam break generated from the scene JSON",
+        "config": {"workload": f"3D WCSPH dam break, {sc.n_flow:,} fluid + {sc.n - sc.n_flow:,} dummy "
+                               f"particles, r=0.0025 m, dt=1e-4, Wendland C2, leap-frog"
+                   if args.scene == "dam_break_3d" else args.scene,
+                   "particles": sc.n, "fluid_particles": sc.n_flow,
+                   "cells": sc.geo.grid_num_total, "parallelism": "single GPU"},
+        "roofline": roof,
+        "kernel_ms_per_step": kernels,
+        "status": {"escaped": status[0], "nan": status[1]},
+        "globals_after": {"front": g[0], "potential": g[1], "kinetic": g[2], "mass": g[3]},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

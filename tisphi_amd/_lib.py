@@ -31,7 +31,8 @@ EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_eng
            "tsph_engine_destroy", "tsph_engine_bytes", "tsph_add_scene", "tsph_remove_scene",
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
            "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_globals",
-           "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_create", "tsph_destroy"]
+           "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats", "tsph_create",
+           "tsph_destroy"]
 
 
 class SceneDesc(ctypes.Structure):
@@ -111,6 +112,7 @@ def load():
         "tsph_status": ([vp, i32, ctypes.POINTER(i64)], i32),
         "tsph_kernel_time": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(i64)], i32),
         "tsph_timing": ([vp, i32], i32),
+        "tsph_neighbor_stats": ([vp, i32, vp, vp, vp], i32),
         "tsph_create": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(vp)], i32),
         "tsph_destroy": ([vp], i32),
     }
@@ -233,6 +235,15 @@ class Engine:
 
     def timing(self, on):
         check(load().tsph_timing(self.h, 1 if on else 0))
+
+    def neighbor_stats(self, slot=0):
+        n = self.n[slot]
+        c = np.zeros(n, dtype=np.int32)
+        s1 = np.zeros(n, dtype=np.int64)
+        s2 = np.zeros(n, dtype=np.int64)
+        check(load().tsph_neighbor_stats(self.h, int(slot), c.ctypes.data, s1.ctypes.data,
+                                         s2.ctypes.data))
+        return c, s1, s2
 
     def kernel_time(self, name):
         ms = ctypes.c_double()

@@ -261,8 +261,8 @@ struct Seq {
     L.go("scan", k_scan_bsum, 1, kScanBlock, E->w.bsum, nb, E->w.start, n);
     L.go("scan", k_scan_down, nb, kScanBlock, E->w.count, (const int32_t*)E->w.bsum, E->w.start, n);
     L.go("scatter", k_scatter, tb(), eb, a);
-    L.go("rank_move", k_rank_move, (E->cap + eb - 1) / eb, eb, a, E->method == kMUI ? 1 : 0,
-         E->method == kMUI ? 1 : 0, E->method == kDP ? 1 : 0);
+    L.go("rank_move", k_rank_move, (E->cap + eb - 1) / eb, eb, a, 1, E->method == kMUI ? 1 : 0,
+         E->method == kDP ? 1 : 0);
     E->parity ^= 1;
   }
   void shepard() { L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args()); }
@@ -1050,6 +1050,38 @@ int tsph_timing(void* engine, int enable) {
   E->timing.acc.clear();
   E->timing.on = enable != 0;
   return TSPH_OK;
+}
+
+int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
+                        int64_t* id_sqsum) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!slot_ok(E, slot) || !count || !id_sum || !id_sqsum) return fail(TSPH_ERR_INVALID_ARG, "bad argument");
+  const SceneDev& S = E->scenes[slot];
+  int32_t* dc = nullptr;
+  long long *d1 = nullptr, *d2 = nullptr;
+  size_t bytes = 0;
+  int r = dalloc(&dc, E->cap, bytes);
+  if (!r) r = dalloc(&d1, E->cap, bytes);
+  if (!r) r = dalloc(&d2, E->cap, bytes);
+  if (!r) {
+    Launcher L{E};
+    if (E->dim == 2)
+      L.go("neighbor_stats", k_neighbor_stats<2>, E->nblocks_tiles(256), 256, E->args(), dc, d1, d2);
+    else
+      L.go("neighbor_stats", k_neighbor_stats<3>, E->nblocks_tiles(256), 256, E->args(), dc, d1, d2);
+    r = L.status;
+  }
+  if (!r) {
+    hipError_t e = hipStreamSynchronize(E->stream);
+    if (e == hipSuccess) e = hipMemcpy(count, dc + S.pbase, S.n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(id_sum, d1 + S.pbase, S.n * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(id_sqsum, d2 + S.pbase, S.n * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = fail(TSPH_ERR_HIP, std::string("neighbor_stats: ") + hipGetErrorString(e));
+  }
+  if (dc) hipFree(dc);
+  if (d1) hipFree(d1);
+  if (d2) hipFree(d2);
+  return r;
 }
 
 int tsph_create(const tsph_scene_desc* desc, void** out_engine) {

@@ -317,7 +317,12 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (a.cur.type[p] != kFluid) return;
+  if (a.cur.type[p] != kFluid) {
+    // Rates of other particles are never written by the reference: zero.
+    a.w.drho[p] = 0.0f;
+    a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
   const SceneDev& S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
@@ -358,7 +363,12 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (a.cur.type[p] != kSoil) return;
+  if (a.cur.type[p] != kSoil) {
+    a.w.drho[p] = 0.0f;
+    a.w.p[p] = 0.0f;
+    a.w.dseq[p] = 0.0;
+    return;
+  }
   const SceneDev& S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
@@ -490,7 +500,10 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (a.cur.type[p] != kSoil) return;
+  if (a.cur.type[p] != kSoil) {
+    a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
   const SceneDev& S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
@@ -527,7 +540,8 @@ __global__ __launch_bounds__(256) void k_mui_adv1(KArgs a) {
 
 // Phase 2 of D2: stress regularisation from the post-advect positions; the
 // stencil centre is the cell of the new position, the pair vectors use the
-// stored (pre-advect) cells.  Result parked in sigt_f.
+// stored (pre-advect) cells.  The result is parked in the other state
+// buffer's stress, which is dead until the next counting sort refills it.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_mui_regu(KArgs a) {
   int s, p;
@@ -554,7 +568,7 @@ __global__ __launch_bounds__(256) void k_mui_regu(KArgs a) {
             });
   const float cs = a.w.cspm[p];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) a.w.sigt_f[c * st + p] = acc[c] * cs;
+  for (int c = 0; c < 6; ++c) a.alt.sig_f[c * st + p] = acc[c] * cs;
 }
 
 __global__ __launch_bounds__(256) void k_mui_adv2(KArgs a) {
@@ -563,7 +577,7 @@ __global__ __launch_bounds__(256) void k_mui_adv2(KArgs a) {
   if (a.cur.type[p] != kSoil) return;
   const int st = a.stride;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) a.cur.sig_f[c * st + p] = a.w.sigt_f[c * st + p];
+  for (int c = 0; c < 6; ++c) a.cur.sig_f[c * st + p] = a.alt.sig_f[c * st + p];
 }
 
 // ======================================================================== DP
@@ -705,7 +719,16 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (a.cur.type[p] != kSoil) return;
+  if (a.cur.type[p] != kSoil) {
+    const int st = a.stride;
+    a.w.drho[p] = 0.0f;
+    a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.w.dsig_d[c * st + p] = 0.0;
+    a.w.dseq[p] = 0.0;
+    a.w.dseqp[p] = 0.0;
+    return;
+  }
   const SceneDev& S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
@@ -997,6 +1020,30 @@ __global__ __launch_bounds__(256) void k_rk_advect(KArgs a) {
     for (int c = 0; c < 6; ++c)
       a.cur.sig_d[c * st + p] += S.dt_d / 6.0 * a.w.dsig_rk[c * st + p];
   }
+}
+
+// Neighbour-set digest for tests: count, sum and sum of squares of the
+// neighbours' id0 (for_all_neighbors membership, particle_system.py:268).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_neighbor_stats(KArgs a, int32_t* __restrict__ cnt,
+                                                        long long* __restrict__ s1,
+                                                        long long* __restrict__ s2) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int32_t* __restrict__ id0 = a.cur.id0;
+  int n = 0;
+  long long a1 = 0, a2 = 0;
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+    const long long id = id0[j];
+    ++n;
+    a1 += id;
+    a2 += id * id;
+  });
+  cnt[p] = n;
+  s1[p] = a1;
+  s2[p] = a2;
 }
 
 // =================================================================== globals

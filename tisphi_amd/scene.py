@@ -116,6 +116,8 @@ class Scene:
 
     @property
     def n(self):
+        if self.pos is None:
+            return int(sum(self.block_counts) + self.n_dummy)
         return int(self.pos.shape[0])
 
     @property
