@@ -183,8 +183,13 @@ int tsph_run_pass(void* engine, int pass_id);
 int tsph_scene_steps(void* engine, int slot, int64_t* out_steps);
 /* Use a captured hipGraph for tsph_step (1, default) or plain launches (0). */
 int tsph_set_graph(void* engine, int enable);
-/* Threads per block for the pair kernels (64, 128, 256, 512, 1024). */
+/* Threads per block of the pair kernels and of the per-particle kernels
+ * (64, 128 or 256). */
 int tsph_set_block(void* engine, int pair_block, int elem_block);
+/* Record per-step neighbour lists in the Shepard walk and reuse them in
+ * the later pair passes of the step (1, default), or walk the cell stencil
+ * in every pass (0).  Both visit the same pairs in the same order. */
+int tsph_set_neighbor_list(void* engine, int enable);
 
 /* ---- reductions --------------------------------------------------------- */
 /* FP64 global quantities of one scene:

@@ -220,6 +220,7 @@ struct Sim {
   // Cell of an offset along one axis: floor((double)o / grid).
   void renorm(int32_t& c, R& o) const {
     double od = (double)o;
+    if (od != od) return;
     double k = std::floor(od / grid);
     double r = od - k * grid;
     if (r >= grid) { k += 1; r -= grid; }

@@ -216,3 +216,20 @@ def test_1000_step_globals_vs_fp64():
     assert m == pytest.approx(7200 * 0.01 ** 2 * 1000, rel=1e-6)
     assert eng.status(slot) == (0, 0)
     eng.close()
+
+
+def test_neighbor_list_path_equals_stencil_walk(case):
+    """Lists recorded in the Shepard walk replay the same pairs in the same
+    order as walking the stencil in every pass: bitwise equal state."""
+    key, sc, orc, H = case
+    res = []
+    for use_nl in (True, False):
+        eng, slot = new_engine(sc)
+        eng.set_neighbor_list(use_nl)
+        copy_state(orc, eng, slot, sc.method)
+        eng.step(3)
+        eng.sync()
+        res.append({f: eng.download(slot, f) for f in state_fields(sc.method)})
+        eng.close()
+    for f in res[0]:
+        np.testing.assert_array_equal(res[0][f], res[1][f], err_msg=f)

@@ -1,0 +1,70 @@
+"""Host-side logic that runs without a GPU: exports, request generation,
+capacity planning, scene families."""
+import numpy as np
+import pytest
+
+from tisphi_amd import export
+from tisphi_amd.batch import build_family, dam_break_family
+from tisphi_amd.serve import plan_capacity, request_mix
+
+
+def _dump(n=50, seed=0):
+    rng = np.random.default_rng(seed)
+    pos = {k: rng.normal(size=n) for k in ("pos.x", "pos.y", "pos.z")}
+    data = {"id0": np.arange(n, dtype=np.int64), "objId": np.zeros(n, np.int64),
+            "material": np.ones(n, np.int64)}
+    for k in export.CSV_COLUMNS[6:]:
+        data[k] = rng.normal(size=n)
+    data["pressure"] = rng.normal(size=n)
+    return pos, data
+
+
+def test_csv_columns_and_header(tmp_path):
+    pos, data = _dump()
+    p = export.export_csv(str(tmp_path / "a.csv"), pos, data)
+    first = open(p).readline().strip()
+    assert first == "# " + ", ".join(export.CSV_COLUMNS)
+    arr = np.loadtxt(p, delimiter=",")
+    assert arr.shape == (50, 17)
+    np.testing.assert_array_equal(arr[:, 0], data["id0"])
+    np.testing.assert_array_equal(arr[:, 3], pos["pos.x"])
+    np.testing.assert_array_equal(arr[:, 16], data["strain_equ"])
+
+
+def test_vtu_round_trip(tmp_path):
+    pos, data = _dump(seed=1)
+    p = export.export_vtu(str(tmp_path / "b"), pos, data)
+    pts, d = export.read_vtu_points(p)
+    np.testing.assert_array_equal(pts[:, 1], pos["pos.y"])
+    for k, v in data.items():
+        np.testing.assert_array_equal(d[k], v)
+
+
+def test_stamps():
+    assert export.step_stamp(42) == "000042"
+    assert export.time_stamp(0.34922) == "time.secx1e6.0349220"
+
+
+def test_dam_break_family_seeded():
+    a = build_family(dam_break_family(16, seed=0))
+    b = build_family(dam_break_family(16, seed=0))
+    assert [s.n for s in a] == [s.n for s in b]
+    for s in a:
+        assert 600 <= s.block_counts[0] <= 7200
+        assert s.n_dummy == 1944
+        assert s.geo.grid_num_total == 6160
+    assert len(set(s.n for s in a)) > 8
+
+
+def test_request_mix_sizes_and_arrivals():
+    reqs = request_mix(n=30, seed=1, rate=2.0, steps_range=(500, 5000))
+    t = [r.arrival for r in reqs]
+    assert t == sorted(t) and t[0] > 0
+    for r in reqs:
+        nf = r.scene.n_flow
+        lo, hi = {"wc2d": (1e3, 5e4), "wc3d": (5e4, 5e5), "mui2d": (1e4, 8e4)}[r.kind]
+        assert 0.7 * lo <= nf <= 1.3 * hi, (r.kind, nf)
+        assert 500 <= r.steps <= 5000
+    cap = plan_capacity(reqs)
+    for (m, d), (p, c) in cap.items():
+        assert p >= max(r.scene.n for r in reqs if (r.scene.method, r.scene.geo.dim) == (m, d))

@@ -30,7 +30,7 @@ PASSES = dict(sort=0, shepard=1, init_tmp=2, one_step=3, lf_half=4, lf_full=5,
 EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_engine_create",
            "tsph_engine_destroy", "tsph_engine_bytes", "tsph_add_scene", "tsph_remove_scene",
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
-           "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_globals",
+           "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list", "tsph_globals",
            "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats", "tsph_create",
            "tsph_destroy"]
 
@@ -108,6 +108,7 @@ def load():
         "tsph_scene_steps": ([vp, i32, ctypes.POINTER(i64)], i32),
         "tsph_set_graph": ([vp, i32], i32),
         "tsph_set_block": ([vp, i32, i32], i32),
+        "tsph_set_neighbor_list": ([vp, i32], i32),
         "tsph_globals": ([vp, i32, dp], i32),
         "tsph_status": ([vp, i32, ctypes.POINTER(i64)], i32),
         "tsph_kernel_time": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(i64)], i32),
@@ -219,6 +220,9 @@ class Engine:
 
     def set_graph(self, on):
         check(load().tsph_set_graph(self.h, 1 if on else 0))
+
+    def set_neighbor_list(self, on):
+        check(load().tsph_set_neighbor_list(self.h, 1 if on else 0))
 
     def set_block(self, pair, elem):
         check(load().tsph_set_block(self.h, int(pair), int(elem)))

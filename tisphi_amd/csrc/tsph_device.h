@@ -88,6 +88,8 @@ struct PWork {
   unsigned long long* status;  // per slot: escaped, NaN
   double* gpart;     // globals partials per tile (4 each)
   double* gout;      // globals per slot (4 each)
+  uint32_t* nl;      // neighbour lists, entry k of particle p at [k*cap + p]
+  int32_t* nlc;      // neighbour count per particle (> nl_cap: walk instead)
 };
 
 struct KArgs {
@@ -98,6 +100,7 @@ struct KArgs {
   int32_t cap;     // particle capacity
   int32_t ccap;    // cell capacity
   int32_t stride;  // component stride of 6-vectors (== cap)
+  int32_t nl_cap;  // neighbour-list capacity per particle
 };
 
 // ------------------------------------------------------------- kernels W
@@ -160,6 +163,7 @@ __device__ __forceinline__ float wc_eos(const SceneDev& S, float rt) {
 // offset renormalised into [0, grid) (same rule as the oracle).
 __device__ __forceinline__ void renorm(int32_t& c, float& o, double grid) {
   double od = (double)o;
+  if (od != od) return;  // NaN offset: keep the cell (the NaN is reported by the step)
   double k = floor(od / grid);
   double r = od - k * grid;
   if (r >= grid) {
@@ -189,7 +193,9 @@ __device__ __forceinline__ void decode_cell(const SceneDev& S, int32_t c, int& x
 // (scx,scy,scz), particles j ascending in each cell, j != i and
 // |x_ij|^2 < r_s^2.  Cells outside the grid are skipped.  (cix,ciy,ciz) is
 // the cell the offset `oi` is measured from (equal to the centre except
-// after advect_pos, before the next hash).  fn(j, x, y, z, r2).
+// after advect_pos, before the next hash).
+// fn(j, x, y, z, r2, m_V_j, code) with code = 9(dx+1) + 3(dy+1) + (dz+1)
+// the stencil cell of j.
 template <int DIM, class Fn>
 __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restrict__ start,
                                      const float4* __restrict__ pos, int32_t shift, int i,
@@ -213,6 +219,7 @@ __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restric
         if (DIM == 3 && (nz < 0 || nz >= gz)) continue;
         const float bz = DIM == 3 ? oiz + (float)(ciz - nz) * S.g : 0.0f;
         const int32_t key = rowkey + nz;
+        const int code = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
         const int32_t jb = start[key] + shift;
         const int32_t je = start[key + 1] + shift;
 #pragma unroll 2
@@ -222,10 +229,37 @@ __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restric
           const float y = by - pj.y;
           const float z = DIM == 3 ? bz - pj.z : 0.0f;
           const float r2 = x * x + y * y + z * z;
-          if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w);
+          if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w, code);
         }
       }
     }
+  }
+}
+
+// Neighbour-list entry: particle index << 5 | stencil code (27 values).
+constexpr int kNlShift = 5;
+__device__ __forceinline__ uint32_t nl_entry(int j, int code) {
+  return ((uint32_t)j << kNlShift) | (uint32_t)code;
+}
+
+// Iterate a particle's stored neighbour list (built from the same walk, so
+// the pairs, their order and the pair vectors are identical to walk()).
+template <int DIM, class Fn>
+__device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __restrict__ nl,
+                                         int n, int stride, const float4* __restrict__ pos, int i,
+                                         float oix, float oiy, float oiz, Fn&& fn) {
+#pragma unroll 2
+  for (int k = 0; k < n; ++k) {
+    const uint32_t e = nl[(size_t)k * stride + i];
+    const int j = (int)(e >> kNlShift);
+    const int code = (int)(e & 31u);
+    const int dx = code / 9 - 1, dy = (code / 3) % 3 - 1, dz = code % 3 - 1;
+    const float4 pj = pos[j];
+    const float x = (oix + (float)(-dx) * S.g) - pj.x;
+    const float y = (oiy + (float)(-dy) * S.g) - pj.y;
+    const float z = DIM == 3 ? (oiz + (float)(-dz) * S.g) - pj.z : 0.0f;
+    const float r2 = x * x + y * y + z * z;
+    fn(j, x, y, z, r2, pj.w, code);
   }
 }
 

@@ -89,6 +89,9 @@ struct tsph_engine {
   size_t bytes = 0;
   int pair_block = 256, elem_block = 256;
   bool use_graph = true;
+  bool use_nl = true;    // build per-step neighbour lists
+  bool nl_valid = false; // lists match the current order and positions
+  int32_t nl_cap = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graph_valid = false;
   int parity = 0;  // which state buffer is current
@@ -113,6 +116,7 @@ struct tsph_engine {
     a.cap = cap;
     a.ccap = ccap;
     a.stride = cap;
+    a.nl_cap = nl_cap;
     return a;
   }
   int nblocks_tiles(int block) const { return (tiles_alloc * 64 + block - 1) / block; }
@@ -166,7 +170,8 @@ int alloc_work(tsph_engine* E) {
       (r = dalloc(&w.bsum, std::max<size_t>(nb, 1), b)) ||
       (r = dalloc(&w.status, 2 * (size_t)E->max_scenes, b)) ||
       (r = dalloc(&w.gpart, 4 * (size_t)E->tiles_alloc, b)) ||
-      (r = dalloc(&w.gout, 4 * (size_t)E->max_scenes, b)))
+      (r = dalloc(&w.gout, 4 * (size_t)E->max_scenes, b)) ||
+      (r = dalloc(&w.nlc, n, b)) || (r = dalloc(&w.nl, (size_t)E->nl_cap * n, b)))
     return r;
   if (E->method != kWC) {
     if ((r = dalloc(&w.sigt_f, n6, b)) || (r = dalloc(&w.dseq, n, b))) return r;
@@ -182,7 +187,7 @@ int alloc_work(tsph_engine* E) {
 void free_work(PWork& w) {
   void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
                   w.dsig_d, w.dseq, w.dseqp, w.xs, w.drho_rk, w.dv_rk, w.dsig_rk, w.count,
-                  w.start, w.bsum, w.status, w.gpart, w.gout};
+                  w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc};
   for (void* p : ptrs)
     if (p) hipFree(p);
   w = PWork{};
@@ -264,24 +269,38 @@ struct Seq {
     L.go("rank_move", k_rank_move, (E->cap + eb - 1) / eb, eb, a, 1, E->method == kMUI ? 1 : 0,
          E->method == kDP ? 1 : 0);
     E->parity ^= 1;
+    E->nl_valid = false;
   }
-  void shepard() { L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args()); }
+  // calc_CSPM_f; with neighbour lists on, the walk also records the lists.
+  void shepard() {
+    if (E->use_nl) {
+      L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args());
+      E->nl_valid = true;
+    } else {
+      L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args());
+    }
+  }
   void init_tmp() { L.go("init_tmp", k_init_tmp, tb(), E->elem_block, E->args()); }
   void one_step() {
+    if (E->nl_valid) one_step_t<true>();
+    else one_step_t<false>();
+  }
+  template <bool NL>
+  void one_step_t() {
     KArgs a = E->args();
     const int B = E->pair_block;
     if (E->method == kWC) {
       L.go("wc_eos", k_wc_eos, tb(), E->elem_block, a);
-      L.go("wc_bdy", k_wc_bdy<DIM>, pb(), B, a);
-      L.go("wc_fluid", k_wc_fluid<DIM>, pb(), B, a);
+      L.go("wc_bdy", k_wc_bdy<DIM, NL>, pb(), B, a);
+      L.go("wc_fluid", k_wc_fluid<DIM, NL>, pb(), B, a);
     } else if (E->method == kMUI) {
-      L.go("mui_soil1", k_mui_soil1<DIM>, pb(), B, a);
-      L.go("soil_bdy", k_soil_bdy<DIM, true>, pb(), B, a);
-      L.go("mui_soil3", k_mui_soil3<DIM>, pb(), B, a);
+      L.go("mui_soil1", k_mui_soil1<DIM, NL>, pb(), B, a);
+      L.go("soil_bdy", k_soil_bdy<DIM, true, NL>, pb(), B, a);
+      L.go("mui_soil3", k_mui_soil3<DIM, NL>, pb(), B, a);
     } else {
       L.go("dp_adapt", k_dp_adapt, tb(), E->elem_block, a);
-      L.go("soil_bdy", k_soil_bdy<DIM, false>, pb(), B, a);
-      L.go("dp_soil", k_dp_soil<DIM>, pb(), B, a);
+      L.go("soil_bdy", k_soil_bdy<DIM, false, NL>, pb(), B, a);
+      L.go("dp_soil", k_dp_soil<DIM, NL>, pb(), B, a);
     }
   }
   void lf_half() { L.go("lf_half", k_lf_half, tb(), E->elem_block, E->args()); }
@@ -309,7 +328,10 @@ struct Seq {
   }
   void advect_pos(bool xsph) {
     KArgs a = E->args();
-    if (xsph) L.go("xsph", k_xsph<DIM>, pb(), E->pair_block, a);
+    if (xsph) {
+      if (E->nl_valid) L.go("xsph", k_xsph<DIM, true>, pb(), E->pair_block, a);
+      else L.go("xsph", k_xsph<DIM, false>, pb(), E->pair_block, a);
+    }
     L.go("advect_pos", k_advect_pos, tb(), E->elem_block, a, xsph ? 1 : 0);
   }
   void advect_something() {
@@ -544,6 +566,7 @@ int h2d(tsph_engine* E, T* dptr, int64_t off, const std::vector<T>& in) {
 
 int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool up) {
   if (!slot_ok(E, slot)) return fail(TSPH_ERR_INVALID_ARG, "bad slot");
+  if (up) E->nl_valid = false;
   if (field < 0 || field >= TSPH_F_COUNT) return fail(TSPH_ERR_INVALID_ARG, "bad field id");
   const SceneDev& S = E->scenes[slot];
   const int64_t n = S.n, off = S.pbase;
@@ -763,7 +786,7 @@ int tsph_engine_create(int method, int dim, int64_t particle_capacity, int64_t c
   *out_engine = nullptr;
   if (method < 1 || method > 3) return fail(TSPH_ERR_INVALID_ARG, "method must be 1, 2 or 3");
   if (dim != 2 && dim != 3) return fail(TSPH_ERR_INVALID_ARG, "dim must be 2 or 3");
-  if (particle_capacity < 1 || particle_capacity > (1ll << 30))
+  if (particle_capacity < 1 || particle_capacity > (1ll << 26))
     return fail(TSPH_ERR_INVALID_ARG, "particle capacity out of range");
   if (cell_capacity < 1 || cell_capacity > (int64_t)kScanTile * kScanTile)
     return fail(TSPH_ERR_INVALID_ARG, "cell capacity out of range");
@@ -780,6 +803,9 @@ int tsph_engine_create(int method, int dim, int64_t particle_capacity, int64_t c
   const int64_t pc = (particle_capacity + kTileAlign - 1) / kTileAlign * kTileAlign + (int64_t)kTileAlign * max_scenes;
   E->cap = (int32_t)pc;
   E->ccap = (int32_t)cell_capacity;
+  // 3D lattice: 92 neighbours (+30 exactly at r_s); 2D: 24 (+4).  Denser
+  // particles fall back to the stencil walk.
+  E->nl_cap = dim == 3 ? 160 : 48;
   E->tiles_alloc = (int32_t)((pc / 64 + 15) / 16 * 16);
   E->palloc.cap = pc;
   E->calloc_.cap = cell_capacity;
@@ -879,6 +905,7 @@ int tsph_add_scene(void* engine, const tsph_scene_desc* desc, int* out_slot) {
   HIP_TRY(hipMemsetAsync(E->w.status + 2 * slot, 0, 16, E->stream));
   E->scenes[slot] = S;
   E->descs[slot] = *desc;
+  E->nl_valid = false;
   E->steps[slot] = 0;
   if ((r = upload_tables(E))) return r;
   *out_slot = slot;
@@ -892,6 +919,7 @@ int tsph_remove_scene(void* engine, int slot) {
   E->palloc.release(E->scenes[slot].pbase);
   E->calloc_.release(E->scenes[slot].cbase);
   E->scenes[slot].active = 0;
+  E->nl_valid = false;
   return upload_tables(E);
 }
 
@@ -988,6 +1016,16 @@ int tsph_set_graph(void* engine, int enable) {
   tsph_engine* E = (tsph_engine*)engine;
   if (!E) return fail(TSPH_ERR_INVALID_ARG, "null engine");
   E->use_graph = enable != 0;
+  return TSPH_OK;
+}
+
+int tsph_set_neighbor_list(void* engine, int enable) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!E) return fail(TSPH_ERR_INVALID_ARG, "null engine");
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  E->use_nl = enable != 0;
+  E->nl_valid = false;
+  invalidate_graph(E);
   return TSPH_OK;
 }
 

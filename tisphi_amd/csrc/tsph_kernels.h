@@ -11,8 +11,17 @@
 
 namespace tsph {
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remap so that every XCD walks one contiguous stretch of tiles:
+// neighbouring particles (and their shared neighbour data) stay in one L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
 __device__ __forceinline__ int wave_id() {
-  return __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  return __builtin_amdgcn_readfirstlane((int)((xcd_block() * blockDim.x + threadIdx.x) >> 6));
 }
 
 // Thread -> (slot, particle).  False for idle lanes.
@@ -235,6 +244,47 @@ __device__ __forceinline__ void walk_c(const SceneDev& S, const KArgs& a, int p,
             C.P.z, fn);
 }
 
+// Neighbours of p: the stored list when it is valid for this step, else the
+// stencil walk.  Both visit the same pairs in the same order.
+template <int DIM, bool NL, class Fn>
+__device__ __forceinline__ void neighbours(const SceneDev& S, const KArgs& a, int p,
+                                           const Centre& C, Fn&& fn) {
+  if (NL) {
+    const int n = a.w.nlc[p];
+    if (n <= a.nl_cap) {
+      for_list<DIM>(S, a.w.nl, n, a.cap, a.cur.pos, p, C.P.x, C.P.y, C.P.z, fn);
+      return;
+    }
+  }
+  walk_c<DIM>(S, a, p, C, fn);
+}
+
+// Neighbour-list build fused with calc_CSPM_f (solver_sph_base.py:386-398):
+// one stencil walk per particle per step; boundary particles keep only
+// their flow neighbours (the only ones their passes read).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_nl_build(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  uint32_t* __restrict__ nl = a.w.nl;
+  const bool bdy = is_bdy(type[p]);
+  const int cap = a.nl_cap, stride = a.cap;
+  int n = 0;
+  float sum = 0.0f;
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int code) {
+    const bool fj = is_flow(type[j]);
+    if (fj) sum += mVj * kernel_W(S, sqrtf(r2));
+    if (bdy && !fj) return;
+    if (n < cap) nl[(size_t)n * stride + p] = nl_entry(j, code);
+    ++n;
+  });
+  a.w.nlc[p] = n;
+  a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
+}
+
 // =================================================================== Shepard
 // calc_CSPM_f (solver_sph_base.py:386-398)
 template <int DIM>
@@ -245,7 +295,7 @@ __global__ __launch_bounds__(256) void k_shepard(KArgs a) {
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   float sum = 0.0f;
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     if (is_flow(type[j])) sum += mVj * kernel_W(S, sqrtf(r2));
   });
   a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
@@ -282,7 +332,7 @@ __global__ __launch_bounds__(256) void k_wc_eos(KArgs a) {  // solver_sph_wc.py:
 }
 
 // Dummy extrapolation (solver_sph_wc.py:90-103; solver_sph_base.py:653-663).
-template <int DIM>
+template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -294,7 +344,7 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   const float* __restrict__ pr = a.w.p;
   const float gy = S.grav[1];
   float svx = 0.f, svy = 0.f, svz = 0.f, sp = 0.f;
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     if (!is_flow(type[j])) return;
     const float w = kernel_W(S, sqrtf(r2));
     const float4 vj = vtmp[j];
@@ -313,7 +363,7 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
 
 // Fluid loop B (solver_sph_wc.py:108-123; tasks solver_sph_base.py:195-203,
 // solver_sph_wc.py:33-71): d_density and d_vel in one walk.
-template <int DIM>
+template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -332,7 +382,7 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   const float pri = pr[p] / (vi.w * vi.w);
   const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
   float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f;
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     const float rn = sqrtf(r2);
     const float f = kernel_gradfac(S, rn);
     const float gx = f * x, gy = f * y, gz = f * z;
@@ -359,7 +409,7 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
 
 // ======================================================================= muI
 // Soil loop 1 (solver_sph_muI.py:200-225).
-template <int DIM>
+template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -375,7 +425,7 @@ __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
   const float4 vi = vtmp[p];
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   float dd = 0.f;
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     const float f = kernel_gradfac(S, sqrtf(r2));
     const float gw[3] = {f * x, f * y, f * z};
     const float4 vj = vtmp[j];
@@ -422,7 +472,7 @@ __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
 }
 
 // Dummy loop of muI (:228-242) and DP (:220-231).
-template <int DIM, bool EXTRAP_DENSITY>
+template <int DIM, bool EXTRAP_DENSITY, bool NL>
 __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -434,7 +484,7 @@ __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
   const float* __restrict__ sgt = a.w.sigt_f;
   const int st = a.stride;
   float sv[3] = {0.f, 0.f, 0.f}, sr = 0.f, ssg[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     if (!is_flow(type[j])) return;
     const float w = kernel_W(S, sqrtf(r2));
     const float4 vj = vtmp[j];
@@ -496,7 +546,7 @@ __device__ __forceinline__ float4 soil_dv_finish(const SceneDev& S, const float 
                      dv[2] + S.grav[2] + damp * vi.z, 0.0f);
 }
 
-template <int DIM>
+template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -515,7 +565,7 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float dv[3] = {0.f, 0.f, 0.f};
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     const float f = kernel_gradfac(S, sqrtf(r2));
     soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vtmp[j].w, f, x, y, z, dv);
   });
@@ -560,7 +610,7 @@ __global__ __launch_bounds__(256) void k_mui_regu(KArgs a) {
   const int ti = type[p];
   float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   walk<DIM>(S, a.w.start, a.cur.pos, C.shift, p, sx, sy, sz, C.cx, C.cy, C.cz, C.P.x, C.P.y,
-            C.P.z, [&](int j, float x, float y, float z, float r2, float mVj) {
+            C.P.z, [&](int j, float x, float y, float z, float r2, float mVj, int) {
               if (type[j] != ti) return;
               const float w = kernel_W(S, sqrtf(r2));
 #pragma unroll
@@ -715,7 +765,7 @@ __device__ void dp_bui2008(const SceneDev& S, const double L[3][3], const double
 
 // DP soil loop (solver_sph_dp.py:237-270): velocity gradient, d_density
 // and the stress acceleration in one walk, then the FP64 stress rate.
-template <int DIM>
+template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -741,7 +791,7 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   float dd = 0.f, dv[3] = {0.f, 0.f, 0.f};
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     const float f = kernel_gradfac(S, sqrtf(r2));
     const float gw[3] = {f * x, f * y, f * z};
     const float4 vj = vtmp[j];
@@ -846,7 +896,7 @@ __global__ __launch_bounds__(256) void k_lf_full(KArgs a) {
 }
 
 // XSPH sum (solver_sph_base.py:223-226), from pre-advect positions (D3).
-template <int DIM>
+template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_xsph(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -858,7 +908,7 @@ __global__ __launch_bounds__(256) void k_xsph(KArgs a) {
   const float4* __restrict__ vel = a.cur.vel;
   const float4 vi = vel[p];
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     if (type[j] != ti) return;
     const float w = kernel_W(S, sqrtf(r2));
     const float4 vj = vel[j];
@@ -1035,7 +1085,7 @@ __global__ __launch_bounds__(256) void k_neighbor_stats(KArgs a, int32_t* __rest
   const int32_t* __restrict__ id0 = a.cur.id0;
   int n = 0;
   long long a1 = 0, a2 = 0;
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj) {
+  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     const long long id = id0[j];
     ++n;
     a1 += id;

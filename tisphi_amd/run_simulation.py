@@ -1,0 +1,94 @@
+"""Headless counterpart of the reference's run_simulation.py + ui_sim loop.
+
+    python -m tisphi_amd.run_simulation --scene_file data/scenes/x.json
+
+Same CLI flag (run_simulation.py:8-11).  The GGUI window is replaced by a
+headless loop with the reference's step semantics (eng/ui_sim.py:99-242):
+``stepsPerRenderUpdate`` steps per frame, ``exitAtStep`` / ``exitAtTime``
+end the run (``cur_time = dt * count_step``), ``stopAtStep`` /
+``stopAtTime`` / ``pauseAtStart`` have no interactive meaning and are
+ignored, exports follow ``exportEveryTime`` / ``exportEveryRender`` with
+``exportCSV`` / ``exportVTK`` (PNG frames need the window and are not
+written).  ``--steps`` caps the run for scenes that set neither exit rule.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+from . import export
+from .eng.configer_builder import SimConfiger
+from .eng.simulation import Simulation
+
+
+def run(scene_path, max_steps=None, out_dir=None, log=print):
+    cfg = SimConfiger(scene_file_path=scene_path)
+    case = Simulation(cfg)
+    g = cfg.get_cfg_default
+    substeps = max(1, int(g("stepsPerRenderUpdate", 1)))
+    exit_at_step = g("exitAtStep", 0)
+    exit_at_time = g("exitAtTime", 0)
+    every_time = g("exportEveryTime", 0)
+    every_render = g("exportEveryRender", 0)
+    want = (g("exportCSV", False) or g("exportVTK", False))
+    judge_path = (every_render > 0 or every_time > 0) and want and out_dir is not None
+    if judge_path:
+        os.makedirs(out_dir, exist_ok=True)
+        with open(os.path.join(out_dir, "_info.txt"), "w") as f:
+            f.write("==== Running Info ====\nTotal particle number: {:,}\ndt={:.6f}s\n\n==== Configure Info ====\n{}"
+                    .format(case.ps.particle_num[None], case.solver.dt[None], json.dumps(cfg.config)))
+    if not (exit_at_step or exit_at_time or max_steps):
+        raise SystemExit("scene sets neither exitAtStep nor exitAtTime: pass --steps")
+    dt = case.solver.dt[None]
+    count_step = 0
+    next_time = every_time
+
+    def do_export(stamp):
+        pos, data = case.ps.dump()
+        if g("exportCSV", False):
+            export.export_csv(os.path.join(out_dir, f"sim.csv.{stamp}.csv"), pos, data)
+        if g("exportVTK", False):
+            export.export_vtu(os.path.join(out_dir, f"sim.vtk.{stamp}"), pos, data)
+
+    if judge_path and every_time > 0:
+        do_export(export.time_stamp(0.0))
+    elif judge_path and every_render > 0:
+        do_export(export.step_stamp(0))
+    t0 = time.perf_counter()
+    while True:
+        n = substeps
+        if max_steps:
+            n = min(n, max_steps - count_step)
+        case.solver.steps(n)
+        count_step += n
+        cur_time = dt * count_step
+        if judge_path:
+            if every_time > 0 and cur_time >= next_time:
+                do_export(export.time_stamp(cur_time))
+                next_time += every_time
+            elif every_render > 0 and count_step % (every_render * substeps) == 0:
+                do_export(export.step_stamp(count_step))
+        if ((exit_at_step and count_step >= exit_at_step) or (exit_at_time and cur_time >= exit_at_time)
+                or (max_steps and count_step >= max_steps)):
+            break
+    case.solver.sync()
+    wall = time.perf_counter() - t0
+    glb = case.ps.globals()
+    log(f"steps {count_step}  sim time {dt * count_step:.6f}s  wall {wall:.3f}s  "
+        f"front {glb[0]:.6f}  PE {glb[1]:.6f}  KE {glb[2]:.6f}  M {glb[3]:.6f}")
+    return case, count_step, glb
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="tiSPHi (MI355X engine, headless)")
+    ap.add_argument("--scene_file", default="", help="scene file")
+    ap.add_argument("--steps", type=int, default=None, help="stop after this many steps")
+    ap.add_argument("--out", default=None, help="export directory")
+    a = ap.parse_args(argv)
+    run(a.scene_file, a.steps, a.out)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

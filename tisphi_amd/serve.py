@@ -1,0 +1,185 @@
+"""Continuous-batching scene server.
+
+Requests (a scene plus a step budget) arrive over time.  The server keeps
+one engine per (method, dimension) class, each sized from an HBM budget and
+driven on its own HIP stream.  Between iterations it admits waiting requests
+into free scene slots / particle / cell ranges (first come, first served per
+class), advances every engine that has residents, and retires scenes whose
+budget is spent.  Particle state stays resident on the device from admission
+to retirement; the engines' launches overlap on the GPU.
+
+Iteration length: each engine advances by the smallest remaining budget of
+its residents (capped by ``chunk``), so every scene stops exactly at its
+budget and results do not depend on what else is resident.
+"""
+import copy
+import json
+import math
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .eng.configer_builder import SimConfiger
+from .scene import build_scene
+
+SCENES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "scenes")
+
+
+@dataclass
+class Request:
+    rid: int
+    scene: object          # tisphi_amd.scene.Scene (generated)
+    steps: int
+    arrival: float         # seconds after the stream starts
+    kind: str = ""
+    admitted: float = None
+    finished: float = None
+    result: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+def _resized(base, n_target, dims):
+    """Scale the particle radius of ``base`` so its block holds ~n_target
+    particles (``dims`` = number of lattice axes)."""
+    c = copy.deepcopy(json.load(open(os.path.join(SCENES, base + ".json"))))
+    size = c["Blocks"][0]["size"]
+    vol = float(np.prod(size[:dims]))
+    d = (vol / n_target) ** (1.0 / dims)
+    c["Configuration"]["particleRadius"] = d / 2
+    return c
+
+
+def request_mix(n=64, seed=1, rate=None, steps_range=(500, 5000), scale_steps=1.0,
+                kinds=("wc2d", "wc3d", "mui2d")):
+    """The served stream: kinds in equal shares, fluid/soil particle counts
+    log-uniform (2D WC 1k-50k, 3D WC 50k-500k, 2D muI 10k-80k), step budgets
+    uniform in ``steps_range`` (times ``scale_steps``), Poisson arrivals at
+    ``rate`` requests/s (all at t=0 when rate is None)."""
+    rng = np.random.default_rng(seed)
+    spec = {"wc2d": ("dam_break_2d", 2, 1e3, 5e4), "wc3d": ("dam_break_3d", 3, 5e4, 5e5),
+            "mui2d": ("column_collapse_mui", 2, 1e4, 8e4)}
+    t = 0.0
+    out = []
+    for k in range(n):
+        kind = kinds[k % len(kinds)] if rng is None else kinds[int(rng.integers(len(kinds)))]
+        base, dims, lo, hi = spec[kind]
+        nt = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        cfg = SimConfiger(config=_resized(base, nt, dims))
+        steps = max(1, int(round(rng.integers(steps_range[0], steps_range[1] + 1) * scale_steps)))
+        if rate:
+            t += rng.exponential(1.0 / rate)
+        out.append(Request(k, build_scene(cfg), steps, t, kind))
+    return out
+
+
+class _Class:
+    def __init__(self, method, dim, particles, cells, max_scenes):
+        self.engine = _lib.Engine(method, dim, particles, cells, max_scenes)
+        self.resident = {}  # slot -> (request, steps done)
+
+
+class Server:
+    def __init__(self, capacity, chunk=100, max_scenes=256):
+        """``capacity``: {(method, dim): (particles, cells)} — the HBM budget
+        split per class (see ``plan_capacity``)."""
+        self.chunk = chunk
+        self.classes = {k: _Class(k[0], k[1], p, c, max_scenes) for k, (p, c) in capacity.items()}
+
+    @property
+    def hbm_bytes(self):
+        return sum(c.engine.bytes for c in self.classes.values())
+
+    def _admit(self, cls, req, now):
+        sc = req.scene
+        try:
+            slot = cls.engine.add_scene(sc.desc())
+        except _lib.TsphError as e:
+            if e.code == _lib.ERR_CAPACITY:
+                return False
+            raise
+        _lib.upload_scene(cls.engine, slot, sc)
+        cls.resident[slot] = [req, 0]
+        req.admitted = now
+        return True
+
+    def run(self, requests, realtime=True):
+        """Serve ``requests`` (sorted by arrival).  Returns them with
+        admission / completion times and final globals filled in."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting = {k: [] for k in self.classes}
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        done = []
+        total_steps = 0
+        while pending or any(waiting.values()) or any(c.resident for c in self.classes.values()):
+            tn = now() if realtime else float("inf")
+            while pending and pending[0].arrival <= tn:
+                r = pending.pop(0)
+                waiting[(r.scene.method, r.scene.geo.dim)].append(r)
+            for k, cls in self.classes.items():
+                q = waiting[k]
+                while q and self._admit(cls, q[0], now()):
+                    q.pop(0)
+            active = [c for c in self.classes.values() if c.resident]
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                    continue
+                if any(waiting.values()):
+                    raise RuntimeError("a waiting request does not fit any engine")
+                break
+            adv = {}
+            for c in active:
+                k = min(min(r.steps - s for r, s in c.resident.values()), self.chunk)
+                c.engine.step(k)  # asynchronous: engines overlap on the device
+                adv[id(c)] = k
+            for c in active:
+                c.engine.sync()
+            for c in active:
+                k = adv[id(c)]
+                for slot in list(c.resident):
+                    r, s = c.resident[slot]
+                    s += k
+                    total_steps += k
+                    c.resident[slot][1] = s
+                    if s >= r.steps:
+                        r.result = {"globals": c.engine.globals(slot), "steps": c.engine.steps(slot)}
+                        r.finished = now()
+                        c.engine.remove_scene(slot)
+                        del c.resident[slot]
+                        done.append(r)
+        self.scene_steps = total_steps
+        self.wall = now()
+        return done
+
+    def close(self):
+        for c in self.classes.values():
+            c.engine.close()
+
+
+def plan_capacity(requests, hbm_budget_bytes=None, headroom=1.0):
+    """Per-class capacity: room for the largest ``headroom`` x concurrent
+    load of the class (by default every request of the class at once),
+    bounded by an HBM budget split in proportion to particle counts."""
+    need = {}
+    for r in requests:
+        k = (r.scene.method, r.scene.geo.dim)
+        p, c = need.get(k, (0, 0))
+        need[k] = (p + r.scene.n, c + r.scene.geo.grid_num_total)
+    cap = {k: (int(p * headroom) + 1, int(c * headroom) + 1) for k, (p, c) in need.items()}
+    if hbm_budget_bytes:
+        per_particle = 1400  # upper bound of engine bytes per particle (3D DP)
+        tot = sum(p for p, _ in cap.values())
+        scale = min(1.0, hbm_budget_bytes / (per_particle * max(tot, 1)))
+        cap = {k: (max(int(p * scale), 1), c) for k, (p, c) in cap.items()}
+    return cap
+
+
+def percentile(v, q):
+    return float(np.percentile(np.asarray(v, dtype=np.float64), q)) if len(v) else float("nan")
