@@ -53,21 +53,54 @@ def read_traffic(path):
     return None
 
 
-def cpu_baseline(scene, steps):
+def cpu_baseline(scene, max_steps, budget_s=20.0):
     """The FP64 oracle (C++/OpenMP restatement of the reference) timed on
-    the host cores on ``steps`` steps of the same scene."""
+    the host cores on a bounded sample of steps of the same scene: one
+    warm-up step, then as many steps as fit in ``budget_s`` (at least one,
+    at most ``max_steps``).  Progress goes to stderr after every step."""
     from oracle import oracle as orc
     cores = min(len(os.sched_getaffinity(0)), 16)
     orc.set_threads(cores)
     o = orc.Oracle(scene.desc(), 64)
     orc.load_scene(o, scene)
-    o.step(1)  # warm-up (first sort of the generated order)
     t0 = time.perf_counter()
-    o.step(steps)
+    o.step(1)  # warm-up (first sort of the generated order)
+    t1 = time.perf_counter() - t0
+    print(f"bench: cpu baseline warm-up step {t1:.1f} s on {cores} threads", file=sys.stderr, flush=True)
+    steps = int(max(1, min(max_steps, budget_s // max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        o.step(1)
+        print(f"bench: cpu baseline step {k + 1}/{steps} at {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     return dict(value=scene.n_flow * steps / dt, unit="particle-steps/s", cores=cores, kind="port",
-                sample=f"{steps} steps of the full workload after 1 warm-up step, FP64 oracle "
-                       f"({dt:.1f} s, {os.cpu_count()} logical CPUs visible)")
+                sample=f"{steps} step(s) of the full workload after 1 warm-up step, FP64 oracle, "
+                       f"{dt:.1f} s, {cores} OpenMP threads of {os.cpu_count()} logical CPUs")
+
+
+def batched_vs_plain(n_scenes=256, steps=100, plain_steps=10):
+    """256 uneven 2D dam breaks (seed 0): scene-steps/s advanced together in
+    one engine (hipGraph replay) and one scene per engine, one after the
+    other (the one-scene-per-launch path, best of eager and graph)."""
+    from tisphi_amd.batch import build_family, dam_break_family, run_batched, run_plain
+    scenes = build_family(dam_break_family(n_scenes, seed=0))
+    t_b, b = run_batched(scenes, steps, graph=True)
+    b.close()
+    print(f"bench: batched {n_scenes} scenes x {steps} steps in {t_b:.3f} s", file=sys.stderr, flush=True)
+    best = None
+    for graph in (False, True):
+        t_p, engs = run_plain(scenes, plain_steps, graph=graph)
+        for e in engs:
+            e.close()
+        rate = n_scenes * plain_steps / t_p
+        best = rate if best is None or rate > best else best
+        print(f"bench: plain graph={graph} {n_scenes} scenes x {plain_steps} steps in {t_p:.3f} s",
+              file=sys.stderr, flush=True)
+    batched = n_scenes * steps / t_b
+    return dict(scenes=n_scenes, particles=sum(s.n for s in scenes),
+                batched_scene_steps_per_s=batched, plain_scene_steps_per_s=best,
+                speedup=batched / best)
 
 
 def main():
@@ -75,10 +108,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scene", default="dam_break_3d")
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -92,6 +126,7 @@ def main():
     from tisphi_amd.scene import build_scene
 
     sc = build_scene(SimConfiger(os.path.join(ROOT, "tisphi_amd", "scenes", args.scene + ".json")))
+    print(f"bench: {sc.n} particles generated", file=sys.stderr, flush=True)
     dim = sc.geo.dim
     eng = L.Engine(sc.method, dim, sc.n, sc.geo.grid_num_total, 1)
     slot = eng.add_scene(sc.desc())
@@ -107,10 +142,11 @@ def main():
     eng.sync()
     t = time.perf_counter() - t0
     value = sc.n_flow * args.steps / t
+    print(f"bench: gpu {args.steps} steps in {t:.3f} s", file=sys.stderr, flush=True)
     ms_pair, n_pair = eng.kernel_time(PAIR_KERNEL) if not args.graph else (0.0, 0)
     kernels = {}
     if not args.graph:
-        for k in ("hash", "scan", "scatter", "rank_move", "shepard", "init_tmp", "wc_eos", "wc_bdy",
+        for k in ("hash", "scan", "scatter", "rank_move", "shepard", "nl_build", "init_tmp", "wc_eos", "wc_bdy",
                   "wc_fluid", "lf_half", "lf_full", "advect_pos", "chk_density"):
             ms, n = eng.kernel_time(k)
             if n:
@@ -148,6 +184,8 @@ def main():
         "status": {"escaped": status[0], "nan": status[1]},
         "globals_after": {"front": g[0], "potential": g[1], "kinetic": g[2], "mass": g[3]},
     }
+    if not args.no_batch:
+        out["batched"] = batched_vs_plain()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

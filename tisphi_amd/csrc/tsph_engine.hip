@@ -276,6 +276,7 @@ struct Seq {
     if (E->use_nl) {
       L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args());
       E->nl_valid = true;
+      L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args());
     } else {
       L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args());
     }

@@ -259,9 +259,11 @@ __device__ __forceinline__ void neighbours(const SceneDev& S, const KArgs& a, in
   walk_c<DIM>(S, a, p, C, fn);
 }
 
-// Neighbour-list build fused with calc_CSPM_f (solver_sph_base.py:386-398):
-// one stencil walk per particle per step; boundary particles keep only
-// their flow neighbours (the only ones their passes read).
+// Neighbour-list build: one stencil walk per particle per step, recording
+// (j << 5 | stencil code) for every neighbour; boundary particles keep only
+// their flow neighbours (the only ones their passes read).  The candidate
+// loop does nothing else, so the rarely taken append is the only divergent
+// work in it.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_nl_build(KArgs a) {
   int s, p;
@@ -273,15 +275,26 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a) {
   const bool bdy = is_bdy(type[p]);
   const int cap = a.nl_cap, stride = a.cap;
   int n = 0;
-  float sum = 0.0f;
   walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int code) {
-    const bool fj = is_flow(type[j]);
-    if (fj) sum += mVj * kernel_W(S, sqrtf(r2));
-    if (bdy && !fj) return;
+    if (bdy && !is_flow(type[j])) return;
     if (n < cap) nl[(size_t)n * stride + p] = nl_entry(j, code);
     ++n;
   });
   a.w.nlc[p] = n;
+}
+
+// calc_CSPM_f (solver_sph_base.py:386-398) over the recorded lists.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_shepard_list(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  float sum = 0.0f;
+  neighbours<DIM, true>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
+    if (is_flow(type[j])) sum += mVj * kernel_W(S, sqrtf(r2));
+  });
   a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
 }
 
