@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--scene", default="dam_break_3d")
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -186,6 +187,9 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched_vs_plain()
+    if not args.no_serve:
+        from tisphi_amd.serve import serve_bench
+        out["served"] = serve_bench(log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

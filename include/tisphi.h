@@ -140,7 +140,7 @@ enum {
   TSPH_PASS_LF_FULL = 5,          /* advect_LF                              */
   TSPH_PASS_ADVECT_POS = 6,       /* advect_pos (+XSPH)                     */
   TSPH_PASS_ADVECT_SOMETHING = 7, /* solver-specific post-update            */
-  TSPH_PASS_STEP = 8,             /* one full step() via the pass sequence  */
+  TSPH_PASS_STEP = 8,             /* one full step(), one launch per pass   */
   TSPH_PASS_SE = 9,               /* advect_SE                              */
   TSPH_PASS_ENFORCE_BOUNDARY = 10 /* enforce_boundary (boundary == 1)       */
 };

@@ -28,7 +28,8 @@ def alone(sc, steps):
 
 def test_batch_alone_equals_in_mix():
     scenes = build_family(dam_break_family(12, seed=3))
-    b = BatchEngine(scenes)
+    # room for one more scene of the family after a retirement
+    b = BatchEngine(scenes, spare_particles=10_000, spare_cells=6160, max_scenes=13)
     b.step(40)
     b.sync()
     for k in (0, 5, 11):

@@ -30,6 +30,10 @@ CASES = {
     "wc2d_xsph_cubic": ("dam_break_2d", {"xsph": True, "kernel": 0}, 100, 0.6),
     "mui3d": ("column_collapse_mui", {"is2D": False, "particleRadius": 0.004,
                                       "domainEnd": [0.6, 0.3, 0.1]}, 20, 0.2),
+    "dp3d": ("column_collapse_dp", {"is2D": False, "particleRadius": 0.004,
+                                    "domainEnd": [0.6, 0.3, 0.1]}, 20, 0.2),
+    "wc2d_se": ("dam_break_2d", {"timeIntegration": 1}, 100, 0.6),
+    "wc2d_rk4_collision": ("dam_break_2d", {"timeIntegration": 4, "boundary": 1}, 60, 0.6),
 }
 
 STATE = ["cell", "offset", "vel", "density", "mass", "m_V", "type", "id0", "obj_id", "pressure",

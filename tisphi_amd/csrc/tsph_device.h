@@ -70,6 +70,7 @@ struct PWork {
   int32_t* tmp;      // scan position -> particle
   int32_t* sid;      // particle -> scene slot (static per scene range)
   float* p;          // pressure
+  float* pr2;        // pressure / density_tmp^2 (WC pressure-force term)
   float* cspm;       // Shepard factor
   float* drho;       // d_density
   float4* dv;        // d_vel
@@ -194,7 +195,7 @@ __device__ __forceinline__ void decode_cell(const SceneDev& S, int32_t c, int& x
 // |x_ij|^2 < r_s^2.  Cells outside the grid are skipped.  (cix,ciy,ciz) is
 // the cell the offset `oi` is measured from (equal to the centre except
 // after advect_pos, before the next hash).
-// fn(j, x, y, z, r2, m_V_j, code) with code = 9(dx+1) + 3(dy+1) + (dz+1)
+// fn(j, x, y, z, r2, m_V_j, code) with code = (dx+1)<<4 | (dy+1)<<2 | (dz+1)
 // the stencil cell of j.
 template <int DIM, class Fn>
 __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restrict__ start,
@@ -219,7 +220,7 @@ __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restric
         if (DIM == 3 && (nz < 0 || nz >= gz)) continue;
         const float bz = DIM == 3 ? oiz + (float)(ciz - nz) * S.g : 0.0f;
         const int32_t key = rowkey + nz;
-        const int code = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+        const int code = ((dx + 1) << 4) | ((dy + 1) << 2) | (dz + 1);
         const int32_t jb = start[key] + shift;
         const int32_t je = start[key + 1] + shift;
 #pragma unroll 2
@@ -236,8 +237,8 @@ __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restric
   }
 }
 
-// Neighbour-list entry: particle index << 5 | stencil code (27 values).
-constexpr int kNlShift = 5;
+// Neighbour-list entry: particle index << 6 | stencil code (2 bits per axis).
+constexpr int kNlShift = 6;
 __device__ __forceinline__ uint32_t nl_entry(int j, int code) {
   return ((uint32_t)j << kNlShift) | (uint32_t)code;
 }
@@ -248,16 +249,21 @@ template <int DIM, class Fn>
 __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __restrict__ nl,
                                          int n, int stride, const float4* __restrict__ pos, int i,
                                          float oix, float oiy, float oiz, Fn&& fn) {
+  // shifted centres oi + (c_i - c_j) g for c_j - c_i = -1, 0, +1 (exact
+  // products, rounded once: the same values the walk forms)
+  const float bx[3] = {oix + S.g, oix, oix + (-1.0f) * S.g};
+  const float by[3] = {oiy + S.g, oiy, oiy + (-1.0f) * S.g};
+  const float bz[3] = {oiz + S.g, oiz, oiz + (-1.0f) * S.g};
 #pragma unroll 2
   for (int k = 0; k < n; ++k) {
     const uint32_t e = nl[(size_t)k * stride + i];
     const int j = (int)(e >> kNlShift);
-    const int code = (int)(e & 31u);
-    const int dx = code / 9 - 1, dy = (code / 3) % 3 - 1, dz = code % 3 - 1;
+    const int code = (int)(e & 63u);
     const float4 pj = pos[j];
-    const float x = (oix + (float)(-dx) * S.g) - pj.x;
-    const float y = (oiy + (float)(-dy) * S.g) - pj.y;
-    const float z = DIM == 3 ? (oiz + (float)(-dz) * S.g) - pj.z : 0.0f;
+    const int cx = code >> 4, cy = (code >> 2) & 3, cz = code & 3;
+    const float x = (cx == 0 ? bx[0] : (cx == 1 ? bx[1] : bx[2])) - pj.x;
+    const float y = (cy == 0 ? by[0] : (cy == 1 ? by[1] : by[2])) - pj.y;
+    const float z = DIM == 3 ? (cz == 0 ? bz[0] : (cz == 1 ? bz[1] : bz[2])) - pj.z : 0.0f;
     const float r2 = x * x + y * y + z * z;
     fn(j, x, y, z, r2, pj.w, code);
   }

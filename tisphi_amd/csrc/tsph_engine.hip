@@ -163,7 +163,8 @@ int alloc_work(tsph_engine* E) {
   const size_t nb = (c + kScanTile - 1) / kScanTile;
   int r;
   if ((r = dalloc(&w.key, n, b)) || (r = dalloc(&w.slot, n, b)) || (r = dalloc(&w.tmp, n, b)) ||
-      (r = dalloc(&w.sid, n, b)) || (r = dalloc(&w.p, n, b)) || (r = dalloc(&w.cspm, n, b)) ||
+      (r = dalloc(&w.sid, n, b)) || (r = dalloc(&w.p, n, b)) || (r = dalloc(&w.pr2, n, b)) ||
+      (r = dalloc(&w.cspm, n, b)) ||
       (r = dalloc(&w.drho, n, b)) || (r = dalloc(&w.dv, n, b)) || (r = dalloc(&w.xs, n, b)) ||
       (r = dalloc(&w.drho_rk, n, b)) || (r = dalloc(&w.dv_rk, n, b)) ||
       (r = dalloc(&w.count, c, b)) || (r = dalloc(&w.start, c + 1, b)) ||
@@ -185,7 +186,7 @@ int alloc_work(tsph_engine* E) {
 }
 
 void free_work(PWork& w) {
-  void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
+  void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.pr2, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
                   w.dsig_d, w.dseq, w.dseqp, w.xs, w.drho_rk, w.dv_rk, w.dsig_rk, w.count,
                   w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc};
   for (void* p : ptrs)
@@ -256,7 +257,9 @@ struct Seq {
   int tb() const { return E->nblocks_tiles(E->elem_block); }
   int pb() const { return E->nblocks_tiles(E->pair_block); }
 
-  void sort() {
+  // update_grid_id + prefix sum + counting_sort; with fuse_init the move
+  // also performs init_real2tmp and the first EOS of the step.
+  void sort(bool fuse_init = false) {
     KArgs a = E->args();
     const int eb = E->elem_block;
     L.go("hash", k_hash<DIM>, tb(), eb, a);
@@ -267,7 +270,7 @@ struct Seq {
     L.go("scan", k_scan_down, nb, kScanBlock, E->w.count, (const int32_t*)E->w.bsum, E->w.start, n);
     L.go("scatter", k_scatter, tb(), eb, a);
     L.go("rank_move", k_rank_move, (E->cap + eb - 1) / eb, eb, a, 1, E->method == kMUI ? 1 : 0,
-         E->method == kDP ? 1 : 0);
+         E->method == kDP ? 1 : 0, fuse_init ? 1 : 0);
     E->parity ^= 1;
     E->nl_valid = false;
   }
@@ -282,16 +285,18 @@ struct Seq {
     }
   }
   void init_tmp() { L.go("init_tmp", k_init_tmp, tb(), E->elem_block, E->args()); }
-  void one_step() {
-    if (E->nl_valid) one_step_t<true>();
-    else one_step_t<false>();
+  // one_step of the method; with_eos=false when the EOS was fused into the
+  // kernel before (the move of the sort, or lf_half).
+  void one_step(bool with_eos = true) {
+    if (E->nl_valid) one_step_t<true>(with_eos);
+    else one_step_t<false>(with_eos);
   }
   template <bool NL>
-  void one_step_t() {
+  void one_step_t(bool with_eos) {
     KArgs a = E->args();
     const int B = E->pair_block;
     if (E->method == kWC) {
-      L.go("wc_eos", k_wc_eos, tb(), E->elem_block, a);
+      if (with_eos) L.go("wc_eos", k_wc_eos, tb(), E->elem_block, a);
       L.go("wc_bdy", k_wc_bdy<DIM, NL>, pb(), B, a);
       L.go("wc_fluid", k_wc_fluid<DIM, NL>, pb(), B, a);
     } else if (E->method == kMUI) {
@@ -304,7 +309,7 @@ struct Seq {
       L.go("dp_soil", k_dp_soil<DIM, NL>, pb(), B, a);
     }
   }
-  void lf_half() { L.go("lf_half", k_lf_half, tb(), E->elem_block, E->args()); }
+  void lf_half(bool eos = false) { L.go("lf_half", k_lf_half, tb(), E->elem_block, E->args(), eos ? 1 : 0); }
   void lf_full() { L.go("lf_full", k_lf_full, tb(), E->elem_block, E->args()); }
   void substep(int ti) {
     if (ti == 1) {
@@ -348,15 +353,44 @@ struct Seq {
     }
   }
   void enforce_boundary() { L.go("enforce_boundary", k_enforce_boundary, tb(), E->elem_block, E->args()); }
-  // SPHBase.step (solver_sph_base.py:41-51); all resident scenes share the
-  // engine's time-integration scheme.
-  void step(int ti, bool xsph, bool collide) {
+  // SPHBase.step (solver_sph_base.py:41-51) as the pass sequence, one
+  // launch per reference kernel.
+  void step_passes(int ti, bool xsph, bool collide) {
     sort();
     shepard();
     init_tmp();
     substep(ti);
     advect_pos(xsph);
     advect_something();
+    if (collide) enforce_boundary();
+  }
+  // The same step with elementwise passes fused into their neighbours
+  // (bitwise identical results, fewer launches); all resident scenes share
+  // the engine's time-integration scheme.
+  void step(int ti, bool xsph, bool collide) {
+    sort(true);
+    shepard();
+    if (ti == 2) {
+      one_step(false);
+      lf_half(true);
+      one_step(false);
+    } else if (ti == 1) {
+      one_step(false);
+    } else {
+      substep(4);
+    }
+    const bool lf_in_finish = ti != 4 && !xsph;
+    KArgs a = E->args();
+    if (xsph) {
+      if (ti != 4) lf_full();
+      if (E->nl_valid) L.go("xsph", k_xsph<DIM, true>, pb(), E->pair_block, a);
+      else L.go("xsph", k_xsph<DIM, false>, pb(), E->pair_block, a);
+    }
+    L.go("finish", k_finish, tb(), E->elem_block, a, lf_in_finish ? 1 : 0, xsph ? 1 : 0);
+    if (E->method == kMUI) {
+      L.go("mui_regu", k_mui_regu<DIM>, pb(), E->pair_block, a);
+      L.go("mui_adv", k_mui_adv2, tb(), E->elem_block, a);
+    }
     if (collide) enforce_boundary();
   }
   void globals() {
@@ -795,13 +829,15 @@ int tsph_engine_create(int method, int dim, int64_t particle_capacity, int64_t c
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(TSPH_ERR_HIP, "no HIP device available");
+  // Particle ranges start on 64-particle tiles; each scene may waste <1 tile.
+  const int64_t pc = (particle_capacity + kTileAlign - 1) / kTileAlign * kTileAlign + (int64_t)kTileAlign * max_scenes;
+  if (pc >= (1ll << 26))
+    return fail(TSPH_ERR_INVALID_ARG, "particle capacity (with per-scene alignment) must stay below 2^26");
   tsph_engine* E = new tsph_engine();
   E->method = method;
   E->dim = dim;
   E->max_scenes = max_scenes;
   hipGetDevice(&E->device);
-  // Particle ranges start on 64-particle tiles; each scene may waste <1 tile.
-  const int64_t pc = (particle_capacity + kTileAlign - 1) / kTileAlign * kTileAlign + (int64_t)kTileAlign * max_scenes;
   E->cap = (int32_t)pc;
   E->ccap = (int32_t)cell_capacity;
   // 3D lattice: 92 neighbours (+30 exactly at r_s); 2D: 24 (+4).  Denser
@@ -992,7 +1028,7 @@ int tsph_run_pass(void* engine, int pass_id) {
       case TSPH_PASS_ADVECT_POS: seq.advect_pos(f.xsph); break;
       case TSPH_PASS_ADVECT_SOMETHING: seq.advect_something(); break;
       case TSPH_PASS_ENFORCE_BOUNDARY: seq.enforce_boundary(); break;
-      case TSPH_PASS_STEP: seq.step(f.ti, f.xsph, f.collide); break;
+      case TSPH_PASS_STEP: seq.step_passes(f.ti, f.xsph, f.collide); break;
       default: L.status = fail(TSPH_ERR_INVALID_ARG, "bad pass id");
     }
   };

@@ -43,9 +43,38 @@ __device__ __forceinline__ int32_t scene_shift(const KArgs& a, const SceneDev& S
 // update_grid_id (particle_system.py:228-236): renormalise the offset,
 // flattened cell id, histogram with an arrival slot per particle.
 template <int DIM>
+__device__ __forceinline__ int32_t hash_one(const KArgs& a, int s, int p);
+template <int DIM>
 __global__ __launch_bounds__(256) void k_hash(KArgs a) {
   int s, p;
-  if (!tile_thread(a, s, p)) return;
+  const bool live = tile_thread(a, s, p);
+  // Runs of equal keys among consecutive lanes share one histogram atomic:
+  // the run's first lane adds the run length and hands out its slots.
+  // The live lanes of a wave are a prefix (a tile is one scene's run of
+  // particles); a wave with none leaves before any cross-lane operation.
+  if (!__any(live)) return;
+  int32_t key = -1;
+  if (live) {
+    key = hash_one<DIM>(a, s, p);
+  }
+  const int lane = (int)(threadIdx.x & 63);
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = live && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  const int first = 63 - __clzll(heads & upto);
+  const unsigned long long above = heads & ~upto;
+  const unsigned long long livemask = __ballot(live);
+  const int nlive = __popcll(livemask);
+  const int next = above ? (__ffsll((long long)above) - 1) : nlive;
+  int base = 0;
+  if (head) base = atomicAdd(&a.w.count[key], next - lane);
+  base = __shfl(base, first, 64);
+  if (live) a.w.slot[p] = base + (lane - first);
+}
+
+template <int DIM>
+__device__ __forceinline__ int32_t hash_one(const KArgs& a, int s, int p) {
   const SceneDev& S = a.scenes[s];
   int cx, cy, cz;
   decode_cell(S, a.cur.cell[p], cx, cy, cz);
@@ -77,7 +106,7 @@ __global__ __launch_bounds__(256) void k_hash(KArgs a) {
   a.cur.pos[p] = P;
   const int32_t key = S.cbase + c;
   a.w.key[p] = key;
-  a.w.slot[p] = atomicAdd(&a.w.count[key], 1);
+  return key;
 }
 
 // Exclusive scan of the cell histogram (the inclusive prefix sum of
@@ -188,8 +217,9 @@ __global__ __launch_bounds__(256) void k_scatter(KArgs a) {
 // counting_sort (particle_system.py:238-252) with a deterministic slot:
 // rank among the particles of the same cell by previous index (the serial
 // reference order), then move the persistent state to the other buffer.
+__device__ __forceinline__ float wc_eos(const SceneDev& S, float rt);
 __global__ __launch_bounds__(256) void k_rank_move(KArgs a, int move_vtmp, int move_mui,
-                                                   int move_dp) {
+                                                   int move_dp, int fuse_init) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.w.start[a.ccap]) return;
   const int p = a.w.tmp[k];
@@ -220,6 +250,31 @@ __global__ __launch_bounds__(256) void k_rank_move(KArgs a, int move_vtmp, int m
     a.alt.seq[d] = a.cur.seq[p];
     a.alt.seqp[d] = a.cur.seqp[p];
     a.alt.flag[d] = a.cur.flag[p];
+  }
+  if (fuse_init) {
+    // init_real2tmp (solver_sph_base.py:67-74) at the new position, and the
+    // Tait pressure the first one_step starts with (solver_sph_wc.py:88).
+    const int t = a.cur.type[p];
+    if (is_real(t)) {
+      const float4 v = a.cur.vel[p];
+      a.alt.vtmp[d] = v;
+      if (S.method == kWC && t == kFluid) {
+        const float pv = wc_eos(S, v.w);
+        a.w.p[d] = pv;
+        a.w.pr2[d] = pv / (v.w * v.w);
+      }
+    }
+    if (t == kSoil && S.method == kMUI) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.w.sigt_f[c * st + d] = a.cur.sig_f[c * st + p];
+    } else if (t == kSoil && S.method == kDP) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double v = a.cur.sig_d[c * st + p];
+        a.w.sigt_d[c * st + d] = v;
+        a.w.sigt_f[c * st + d] = (float)v;
+      }
+    }
   }
 }
 
@@ -341,7 +396,12 @@ __global__ __launch_bounds__(256) void k_init_tmp(KArgs a) {
 __global__ __launch_bounds__(256) void k_wc_eos(KArgs a) {  // solver_sph_wc.py:86-88
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (a.cur.type[p] == kFluid) a.w.p[p] = wc_eos(a.scenes[s], a.cur.vtmp[p].w);
+  if (a.cur.type[p] == kFluid) {
+    const float rt = a.cur.vtmp[p].w;
+    const float pv = wc_eos(a.scenes[s], rt);
+    a.w.p[p] = pv;
+    a.w.pr2[p] = pv / (rt * rt);
+  }
 }
 
 // Dummy extrapolation (solver_sph_wc.py:90-103; solver_sph_base.py:653-663).
@@ -371,7 +431,9 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   a.cur.vtmp[p] = make_float4(2.0f * v.x - svx * cs, 2.0f * v.y - svy * cs, 2.0f * v.z - svz * cs,
                               S.rho0);
   const float pv = sp * cs;
-  a.w.p[p] = pv > 0.0f ? pv : 0.0f;
+  const float pc = pv > 0.0f ? pv : 0.0f;
+  a.w.p[p] = pc;
+  a.w.pr2[p] = pc / (S.rho0 * S.rho0);
 }
 
 // Fluid loop B (solver_sph_wc.py:108-123; tasks solver_sph_base.py:195-203,
@@ -390,9 +452,9 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   const float4* __restrict__ vtmp = a.cur.vtmp;
-  const float* __restrict__ pr = a.w.p;
+  const float* __restrict__ pr2 = a.w.pr2;
   const float4 vi = vtmp[p];
-  const float pri = pr[p] / (vi.w * vi.w);
+  const float pri = pr2[p];
   const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
   float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f;
   neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
@@ -411,7 +473,7 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
       if (tj != kFluid) aa = aa * rho0 / vi.w;
       vis = aa * mn / (rn * rn + c2);
     }
-    const float pc = (-rho0 * mVj) * (pri + pr[j] / (vj.w * vj.w));
+    const float pc = (-rho0 * mVj) * (pri + pr2[j]);
     dvx += vis * gx + pc * gx;
     dvy += vis * gy + pc * gy;
     dvz += vis * gz + pc * gz;
@@ -857,7 +919,7 @@ __global__ __launch_bounds__(256) void k_dp_adv(KArgs a) {
 
 // =============================================================== integrators
 // advect_LF_half (solver_sph_base.py:96-104).
-__global__ __launch_bounds__(256) void k_lf_half(KArgs a) {
+__global__ __launch_bounds__(256) void k_lf_half(KArgs a, int eos) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   const SceneDev& S = a.scenes[s];
@@ -871,6 +933,12 @@ __global__ __launch_bounds__(256) void k_lf_half(KArgs a) {
     vt.y += S.half_dt * d.y;
     vt.z += S.half_dt * d.z;
     a.cur.vtmp[p] = vt;
+    // the EOS of the next one_step (solver_sph_wc.py:88), fused
+    if (eos && S.method == kWC && t == kFluid) {
+      const float pv = wc_eos(S, vt.w);
+      a.w.p[p] = pv;
+      a.w.pr2[p] = pv / (vt.w * vt.w);
+    }
   }
   if (t == kSoil && S.method == kDP) {
     const int st = a.stride;
@@ -905,6 +973,72 @@ __global__ __launch_bounds__(256) void k_lf_full(KArgs a) {
     const int st = a.stride;
 #pragma unroll
     for (int c = 0; c < 6; ++c) a.cur.sig_d[c * st + p] += S.dt_d * a.w.dsig_d[c * st + p];
+  }
+}
+
+// End of step, fused: advect_LF (unless already run, XSPH case) ->
+// advect_pos -> the elementwise part of advect_something (WC chk_density,
+// DP return mapping, muI phase 1).  Same per-particle arithmetic as the
+// separate passes.
+__global__ __launch_bounds__(256) void k_finish(KArgs a, int do_lf, int use_xsph) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev& S = a.scenes[s];
+  const int t = a.cur.type[p];
+  const int st = a.stride;
+  float4 v = a.cur.vel[p];
+  float mv = a.cur.pos[p].w;
+  if (do_lf) {
+    if (is_real(t)) {
+      const float4 d = a.w.dv[p];
+      v.w += S.dt * a.w.drho[p];
+      mv = a.cur.mass[p] / v.w;
+      v.x += S.dt * d.x;
+      v.y += S.dt * d.y;
+      v.z += S.dt * d.z;
+      if (isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w)) atomicAdd(&a.w.status[2 * s + 1], 1ull);
+    }
+    if (t == kSoil && S.method == kDP) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.cur.sig_d[c * st + p] += S.dt_d * a.w.dsig_d[c * st + p];
+    }
+  }
+  float4 P = a.cur.pos[p];
+  if (is_real(t)) {
+    if (use_xsph && S.xsph) {
+      const float4 xs = a.w.xs[p];
+      P.x += S.dt * (v.x + 0.5f * xs.x);
+      P.y += S.dt * (v.y + 0.5f * xs.y);
+      P.z += S.dt * (v.z + 0.5f * xs.z);
+    } else {
+      P.x += S.dt * v.x;
+      P.y += S.dt * v.y;
+      P.z += S.dt * v.z;
+    }
+    if (S.dim == 2) P.z = 0.0f;
+  }
+  if ((S.method == kWC && t == kFluid) || (S.method != kWC && t == kSoil)) {
+    v.w = v.w > S.rho0 ? v.w : S.rho0;
+    mv = a.cur.mass[p] / v.w;
+  }
+  P.w = mv;
+  a.cur.vel[p] = v;
+  a.cur.pos[p] = P;
+  if (t != kSoil) return;
+  if (S.method == kMUI) {
+    a.cur.seq[p] = (double)((float)a.cur.seq[p] + S.dt * (float)a.w.dseq[p]);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.cur.sig_f[c * st + p] = a.w.sigt_f[c * st + p];
+  } else if (S.method == kDP) {
+    double q[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) q[c] = a.cur.sig_d[c * st + p];
+    a.cur.flag[p] = dp_flag(S, q);
+    dp_adapt(S, q);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.cur.sig_d[c * st + p] = q[c];
+    a.cur.seq[p] += S.dt_d * a.w.dseq[p];
+    a.cur.seqp[p] += S.dt_d * a.w.dseqp[p];
   }
 }
 

@@ -67,7 +67,7 @@ def request_mix(n=64, seed=1, rate=None, steps_range=(500, 5000), scale_steps=1.
     t = 0.0
     out = []
     for k in range(n):
-        kind = kinds[k % len(kinds)] if rng is None else kinds[int(rng.integers(len(kinds)))]
+        kind = kinds[int(rng.integers(len(kinds)))]
         base, dims, lo, hi = spec[kind]
         nt = math.exp(rng.uniform(math.log(lo), math.log(hi)))
         cfg = SimConfiger(config=_resized(base, nt, dims))
@@ -167,19 +167,78 @@ def plan_capacity(requests, hbm_budget_bytes=None, headroom=1.0):
     """Per-class capacity: room for the largest ``headroom`` x concurrent
     load of the class (by default every request of the class at once),
     bounded by an HBM budget split in proportion to particle counts."""
-    need = {}
+    need, big = {}, {}
     for r in requests:
         k = (r.scene.method, r.scene.geo.dim)
         p, c = need.get(k, (0, 0))
         need[k] = (p + r.scene.n, c + r.scene.geo.grid_num_total)
-    cap = {k: (int(p * headroom) + 1, int(c * headroom) + 1) for k, (p, c) in need.items()}
+        bp, bc = big.get(k, (0, 0))
+        big[k] = (max(bp, r.scene.n), max(bc, r.scene.geo.grid_num_total))
+    # every class can hold at least its largest scene
+    cap = {k: (max(int(p * headroom), big[k][0]) + 64, max(int(c * headroom), big[k][1]) + 1)
+           for k, (p, c) in need.items()}
     if hbm_budget_bytes:
         per_particle = 1400  # upper bound of engine bytes per particle (3D DP)
         tot = sum(p for p, _ in cap.values())
         scale = min(1.0, hbm_budget_bytes / (per_particle * max(tot, 1)))
-        cap = {k: (max(int(p * scale), 1), c) for k, (p, c) in cap.items()}
+        cap = {k: (max(int(p * scale), big[k][0] + 64), c) for k, (p, c) in cap.items()}
     return cap
 
 
 def percentile(v, q):
     return float(np.percentile(np.asarray(v, dtype=np.float64), q)) if len(v) else float("nan")
+
+
+def plain_rate(requests, steps=10):
+    """Scene-steps/s of the one-scene-per-launch path on the same mix: each
+    request alone in its own engine, stepped and synchronised one after the
+    other (``steps`` steps each, after one warm-up step)."""
+    total_t = 0.0
+    for r in requests:
+        sc = r.scene
+        e = _lib.Engine(sc.method, sc.geo.dim, sc.n, sc.geo.grid_num_total, 1)
+        s = e.add_scene(sc.desc())
+        _lib.upload_scene(e, s, sc)
+        e.step(1)
+        e.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e.step(1)
+            e.sync()
+        total_t += time.perf_counter() - t0
+        e.close()
+    return len(requests) * steps / total_t
+
+
+def serve_bench(n=24, seed=1, load=0.7, chunk=100, log=None):
+    """The served-stream measurement: capacity from serving the whole mix at
+    once, then a Poisson stream at ``load`` x capacity; returns throughput,
+    latency percentiles and the plain path's rate on the same requests."""
+    say = log or (lambda *a: None)
+    reqs = request_mix(n=n, seed=seed, rate=None)
+    cap = plan_capacity(reqs)
+    srv = Server(cap, chunk=chunk, max_scenes=n)
+    done = srv.run(reqs, realtime=False)
+    t_all = srv.wall
+    steps_all = sum(r.steps for r in done)
+    capacity = n / t_all
+    say(f"serve: all {n} requests at once in {t_all:.2f} s ({steps_all / t_all:.0f} scene-steps/s)")
+    rate = load * capacity
+    reqs2 = request_mix(n=n, seed=seed, rate=rate)
+    srv2 = Server(cap, chunk=chunk, max_scenes=n)
+    done2 = srv2.run(reqs2, realtime=True)
+    lat = [r.latency for r in done2]
+    say(f"serve: Poisson stream at {rate:.3f} req/s done in {srv2.wall:.2f} s")
+    plain = plain_rate(reqs[: min(n, 12)])
+    say(f"serve: plain path {plain:.1f} scene-steps/s")
+    out = dict(requests=n, arrival_rate_per_s=rate, load=load,
+               capacity_requests_per_s=capacity,
+               saturated_scene_steps_per_s=steps_all / t_all,
+               served_scene_steps_per_s=sum(r.steps for r in done2) / srv2.wall,
+               p50_latency_s=percentile(lat, 50), p99_latency_s=percentile(lat, 99),
+               plain_scene_steps_per_s=plain,
+               speedup_vs_plain=(steps_all / t_all) / plain,
+               hbm_bytes=srv.hbm_bytes)
+    srv.close()
+    srv2.close()
+    return out
