@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--nl", type=int, default=None, choices=(0, 1, 2),
+                    help="fix the neighbour-search mode (default: autotuned with the block sizes)")
+    ap.add_argument("--block", type=str, default=None, help="fix pair,elem block sizes, e.g. 256,256")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -129,12 +132,32 @@ def main():
     sc = build_scene(SimConfiger(os.path.join(ROOT, "tisphi_amd", "scenes", args.scene + ".json")))
     print(f"bench: {sc.n} particles generated", file=sys.stderr, flush=True)
     dim = sc.geo.dim
+    say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)  # noqa: E731
     eng = L.Engine(sc.method, dim, sc.n, sc.geo.grid_num_total, 1)
+    say(f"engine holds {eng.bytes / 2**30:.2f} GiB of device memory")
     slot = eng.add_scene(sc.desc())
     L.upload_scene(eng, slot, sc)
+    say("scene uploaded")
     eng.set_graph(args.graph)
+    # first steps one at a time, each reported (the first also captures graphs)
+    for k in range(2):
+        t1 = time.perf_counter()
+        eng.step(1)
+        eng.sync()
+        say(f"step {k + 1} in {1e3 * (time.perf_counter() - t1):.1f} ms")
+    if args.nl is None and args.block is None:
+        # launch shape: bitwise-neutral, picked by timing real steps (untimed)
+        from tisphi_amd.autotune import tune
+        tuned = tune(eng, steps=2, quick=True, use_cache=False, log=say)
+        shape = tuned["shape"]
+    else:
+        pair, elem = (int(v) for v in (args.block or "256,256").split(","))
+        shape = (pair, elem, 1 if args.nl is None else args.nl)
+        eng.set_block(pair, elem)
+        eng.set_neighbor_list(shape[2])
     eng.step(args.warmup)
     eng.sync()
+    say(f"{args.warmup} warm-up steps done")
     if not args.graph:
         eng.timing(True)  # HIP events around every launch on the engine stream
     eng.sync()
@@ -179,7 +202,9 @@ def main():
                                f"particles, r=0.0025 m, dt=1e-4, Wendland C2, leap-frog"
                    if args.scene == "dam_break_3d" else args.scene,
                    "particles": sc.n, "fluid_particles": sc.n_flow,
-                   "cells": sc.geo.grid_num_total, "parallelism": "single GPU"},
+                   "cells": sc.geo.grid_num_total, "parallelism": "single GPU",
+                   "launch_shape": {"pair_block": shape[0], "elem_block": shape[1],
+                                    "neighbour_lists": ["walk", "global", "lds"][shape[2]]}},
         "roofline": roof,
         "kernel_ms_per_step": kernels,
         "status": {"escaped": status[0], "nan": status[1]},

@@ -153,7 +153,10 @@ int tsph_device_count(int* n);
 
 /* ---- engine lifecycle --------------------------------------------------- */
 /* Engine for scenes of one method/dim.  Capacities bound the sum over
- * scenes of particles and of grid cells. */
+ * scenes of particles and of grid cells.  Replaces the Taichi field
+ * allocation of ParticleSystem.__init__ (eng/particle_system.py:12-133) and
+ * the ti.init device-memory reservation (run_simulation.py:23), sized once for
+ * every scene the engine will hold. */
 int tsph_engine_create(int method, int dim, int64_t particle_capacity,
                        int64_t cell_capacity, int max_scenes,
                        void** out_engine);
@@ -162,37 +165,54 @@ int tsph_engine_destroy(void* engine);
 int tsph_engine_bytes(void* engine, int64_t* out_bytes);
 
 /* ---- scenes ----------------------------------------------------------- */
-/* Reserve a slot and its particle/cell ranges; fields start at zero. */
+/* Reserve a slot and its particle/cell ranges; fields start at zero.
+ * Replaces Simulation.__init__ / build_solver (eng/simulation.py:8-22): the
+ * solver type and its constants come from desc. */
 int tsph_add_scene(void* engine, const tsph_scene_desc* desc, int* out_slot);
+/* Free a slot (no reference counterpart: the reference runs one scene per
+ * process). */
 int tsph_remove_scene(void* engine, int slot);
+/* Replace / read one particle field of a slot.  Upload replaces
+ * ParticleSystem.add_particle / _add_particles (eng/particle_system.py:275-319)
+ * and set_id0 (:208-215); download replaces the copy_to_numpy* kernels
+ * (:413-457) and the .to_numpy() reads of dump() (:459-545). */
 int tsph_upload(void* engine, int slot, int field, const void* host,
                 size_t bytes);
 int tsph_download(void* engine, int slot, int field, void* host,
                   size_t bytes);
 
 /* ---- stepping ----------------------------------------------------------- */
-/* Advance every active scene by nsteps reference steps (step(),
- * solver_sph_base.py:41-51).  Asynchronous on the engine stream; a scene
- * leaves the stepping set with tsph_remove_scene. */
+/* Advance every active scene by nsteps reference steps (SPHBase.step,
+ * eng/solver_sph_base.py:41-51: initialize_particle_system, calc_CSPM_f,
+ * substep, advect_pos, advect_something, enforce_boundary).  Asynchronous on
+ * the engine stream; a scene leaves the stepping set with tsph_remove_scene. */
 int tsph_step(void* engine, int nsteps);
 /* Wait for the engine stream; reports escaped / NaN particles. */
 int tsph_sync(void* engine);
-/* Run one pass on every active scene (single-pass parity hooks). */
+/* Run one pass on every active scene (single-pass parity hooks): the
+ * @ti.kernel methods listed with TSPH_PASS_* (eng/particle_system.py:228-258,
+ * eng/solver_sph_base.py:63-245, 393-398, 525-597, and the solvers'
+ * one_step / advect_something). */
 int tsph_run_pass(void* engine, int pass_id);
-/* Steps taken by a slot since it was added. */
+/* Steps taken by a slot since it was added (the step counter of
+ * eng/ui_sim.py's main loop). */
 int tsph_scene_steps(void* engine, int slot, int64_t* out_steps);
 /* Use a captured hipGraph for tsph_step (1, default) or plain launches (0). */
 int tsph_set_graph(void* engine, int enable);
 /* Threads per block of the pair kernels and of the per-particle kernels
  * (64, 128 or 256). */
 int tsph_set_block(void* engine, int pair_block, int elem_block);
-/* Record per-step neighbour lists in the Shepard walk and reuse them in
- * the later pair passes of the step (1, default), or walk the cell stencil
- * in every pass (0).  Both visit the same pairs in the same order. */
+/* Neighbour search mode: 1 (default) records per-step neighbour lists in
+ * one stencil walk and replays them in the later pair passes of the step;
+ * 2 does the same with the walk reading LDS-staged cell windows; 0 walks
+ * the cell stencil in every pass.  All visit the same pairs in the same
+ * order (for_all_neighbors, eng/particle_system.py:259-269). */
 int tsph_set_neighbor_list(void* engine, int enable);
 
 /* ---- reductions --------------------------------------------------------- */
-/* FP64 global quantities of one scene:
+/* FP64 global quantities of one scene (no reference counterpart; the
+ * reference reads these from dumped fields on the host, e.g. v_maxmin,
+ * eng/particle_system.py:387-397):
  * out[0] surge front = max x over flow particles,
  * out[1] potential energy = sum m * (-g . x),
  * out[2] kinetic energy   = sum 1/2 m |v|^2,
@@ -217,7 +237,9 @@ int tsph_timing(void* engine, int enable);
 int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
                         int64_t* id_sqsum);
 
-/* ---- single-scene convenience (slot 0 of a one-slot engine) ------------ */
+/* ---- single-scene convenience (slot 0 of a one-slot engine) ------------
+ * The reference's one-scene ParticleSystem + solver pair
+ * (eng/simulation.py:8-22) as one handle. */
 int tsph_create(const tsph_scene_desc* desc, void** out_engine);
 int tsph_destroy(void* engine);
 

@@ -223,11 +223,12 @@ def test_1000_step_globals_vs_fp64():
 
 
 def test_neighbor_list_path_equals_stencil_walk(case):
-    """Lists recorded in the Shepard walk replay the same pairs in the same
-    order as walking the stencil in every pass: bitwise equal state."""
+    """Lists recorded in one stencil walk per step (from global memory, or
+    from LDS-staged cell windows) replay the same pairs in the same order as
+    walking the stencil in every pass: bitwise equal state."""
     key, sc, orc, H = case
     res = []
-    for use_nl in (True, False):
+    for use_nl in (0, 1, 2):
         eng, slot = new_engine(sc)
         eng.set_neighbor_list(use_nl)
         copy_state(orc, eng, slot, sc.method)
@@ -235,5 +236,6 @@ def test_neighbor_list_path_equals_stencil_walk(case):
         eng.sync()
         res.append({f: eng.download(slot, f) for f in state_fields(sc.method)})
         eng.close()
-    for f in res[0]:
-        np.testing.assert_array_equal(res[0][f], res[1][f], err_msg=f)
+    for r in res[1:]:
+        for f in res[0]:
+            np.testing.assert_array_equal(r[f], res[0][f], err_msg=f)

@@ -221,8 +221,10 @@ class Engine:
     def set_graph(self, on):
         check(load().tsph_set_graph(self.h, 1 if on else 0))
 
-    def set_neighbor_list(self, on):
-        check(load().tsph_set_neighbor_list(self.h, 1 if on else 0))
+    def set_neighbor_list(self, mode):
+        """0/False: stencil walk in every pass; 1/True: per-step lists;
+        2: per-step lists built from LDS-staged cell windows."""
+        check(load().tsph_set_neighbor_list(self.h, int(mode)))
 
     def set_block(self, pair, elem):
         check(load().tsph_set_block(self.h, int(pair), int(elem)))

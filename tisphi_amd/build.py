@@ -15,6 +15,10 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # Packed FP32 (v_pk_*_f32) off: plain scalar VALU FP32 only.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # No implicit mul+add contraction: every product and sum rounds as
+    # written (fmaf only where the source says so), so one formula gives the
+    # same bits in every kernel that inlines it and in the oracle.
+    "-ffp-contract=off",
     "-Wno-unused-result",
 ]
 

@@ -84,6 +84,7 @@ struct PWork {
   float4* dv_rk;
   double* dsig_rk;
   int32_t* count;    // cell histogram (cell capacity)
+  uint8_t* cflow;    // 1 if the cell holds a flow (fluid/soil) particle
   int32_t* start;    // exclusive scan (cell capacity + 1)
   int32_t* bsum;     // scan block sums
   unsigned long long* status;  // per slot: escaped, NaN
@@ -242,13 +243,21 @@ constexpr int kNlShift = 6;
 __device__ __forceinline__ uint32_t nl_entry(int j, int code) {
   return ((uint32_t)j << kNlShift) | (uint32_t)code;
 }
+// Storage of the lists, blocked per 64-particle tile: entry k of particle p
+// sits at nl_base(p) + 64*k.  A wave's lists are one contiguous run of
+// 64*nl_cap entries whatever the engine capacity, so its list reads and
+// its (lane-divergent) list writes stay within a page or two.
+__device__ __forceinline__ size_t nl_base(int p, int nl_cap) {
+  return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + (size_t)(p & 63);
+}
 
 // Iterate a particle's stored neighbour list (built from the same walk, so
 // the pairs, their order and the pair vectors are identical to walk()).
 template <int DIM, class Fn>
 __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __restrict__ nl,
-                                         int n, int stride, const float4* __restrict__ pos, int i,
+                                         int n, int nl_cap, const float4* __restrict__ pos, int i,
                                          float oix, float oiy, float oiz, Fn&& fn) {
+  const uint32_t* __restrict__ li = nl + nl_base(i, nl_cap);
   // shifted centres oi + (c_i - c_j) g for c_j - c_i = -1, 0, +1 (exact
   // products, rounded once: the same values the walk forms)
   const float bx[3] = {oix + S.g, oix, oix + (-1.0f) * S.g};
@@ -256,7 +265,7 @@ __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __re
   const float bz[3] = {oiz + S.g, oiz, oiz + (-1.0f) * S.g};
 #pragma unroll 2
   for (int k = 0; k < n; ++k) {
-    const uint32_t e = nl[(size_t)k * stride + i];
+    const uint32_t e = li[(size_t)k << 6];
     const int j = (int)(e >> kNlShift);
     const int code = (int)(e & 63u);
     const float4 pj = pos[j];

@@ -90,7 +90,9 @@ struct tsph_engine {
   int pair_block = 256, elem_block = 256;
   bool use_graph = true;
   bool use_nl = true;    // build per-step neighbour lists
+  bool nl_lds = false;   // build them from LDS-staged cell windows
   bool nl_valid = false; // lists match the current order and positions
+  bool cflow_valid = false;  // per-cell flow flags match the current order
   int32_t nl_cap = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graph_valid = false;
@@ -168,11 +170,12 @@ int alloc_work(tsph_engine* E) {
       (r = dalloc(&w.drho, n, b)) || (r = dalloc(&w.dv, n, b)) || (r = dalloc(&w.xs, n, b)) ||
       (r = dalloc(&w.drho_rk, n, b)) || (r = dalloc(&w.dv_rk, n, b)) ||
       (r = dalloc(&w.count, c, b)) || (r = dalloc(&w.start, c + 1, b)) ||
+      (r = dalloc(&w.cflow, c, b)) ||
       (r = dalloc(&w.bsum, std::max<size_t>(nb, 1), b)) ||
       (r = dalloc(&w.status, 2 * (size_t)E->max_scenes, b)) ||
       (r = dalloc(&w.gpart, 4 * (size_t)E->tiles_alloc, b)) ||
       (r = dalloc(&w.gout, 4 * (size_t)E->max_scenes, b)) ||
-      (r = dalloc(&w.nlc, n, b)) || (r = dalloc(&w.nl, (size_t)E->nl_cap * n, b)))
+      (r = dalloc(&w.nlc, n, b)) || (r = dalloc(&w.nl, (size_t)E->nl_cap * 64 * (size_t)E->tiles_alloc, b)))
     return r;
   if (E->method != kWC) {
     if ((r = dalloc(&w.sigt_f, n6, b)) || (r = dalloc(&w.dseq, n, b))) return r;
@@ -188,7 +191,7 @@ int alloc_work(tsph_engine* E) {
 void free_work(PWork& w) {
   void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.pr2, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
                   w.dsig_d, w.dseq, w.dseqp, w.xs, w.drho_rk, w.dv_rk, w.dsig_rk, w.count,
-                  w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc};
+                  w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc, w.cflow};
   for (void* p : ptrs)
     if (p) hipFree(p);
   w = PWork{};
@@ -267,17 +270,22 @@ struct Seq {
     const int nb = (n + kScanTile - 1) / kScanTile;
     L.go("scan", k_scan_reduce, nb, kScanBlock, (const int32_t*)E->w.count, E->w.bsum, n);
     L.go("scan", k_scan_bsum, 1, kScanBlock, E->w.bsum, nb, E->w.start, n);
-    L.go("scan", k_scan_down, nb, kScanBlock, E->w.count, (const int32_t*)E->w.bsum, E->w.start, n);
+    L.go("scan", k_scan_down, nb, kScanBlock, E->w.count, (const int32_t*)E->w.bsum, E->w.start,
+         E->w.cflow, n);
     L.go("scatter", k_scatter, tb(), eb, a);
     L.go("rank_move", k_rank_move, (E->cap + eb - 1) / eb, eb, a, 1, E->method == kMUI ? 1 : 0,
          E->method == kDP ? 1 : 0, fuse_init ? 1 : 0);
     E->parity ^= 1;
     E->nl_valid = false;
+    E->cflow_valid = true;
   }
   // calc_CSPM_f; with neighbour lists on, the walk also records the lists.
   void shepard() {
     if (E->use_nl) {
-      L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args());
+      if (E->nl_lds)
+        L.go("nl_build", k_nl_build_lds<DIM>, E->tiles_alloc, 64, E->args(), E->cflow_valid ? 1 : 0);
+      else
+        L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args(), E->cflow_valid ? 1 : 0);
       E->nl_valid = true;
       L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args());
     } else {
@@ -601,7 +609,7 @@ int h2d(tsph_engine* E, T* dptr, int64_t off, const std::vector<T>& in) {
 
 int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool up) {
   if (!slot_ok(E, slot)) return fail(TSPH_ERR_INVALID_ARG, "bad slot");
-  if (up) E->nl_valid = false;
+  if (up) E->nl_valid = E->cflow_valid = false;
   if (field < 0 || field >= TSPH_F_COUNT) return fail(TSPH_ERR_INVALID_ARG, "bad field id");
   const SceneDev& S = E->scenes[slot];
   const int64_t n = S.n, off = S.pbase;
@@ -942,7 +950,7 @@ int tsph_add_scene(void* engine, const tsph_scene_desc* desc, int* out_slot) {
   HIP_TRY(hipMemsetAsync(E->w.status + 2 * slot, 0, 16, E->stream));
   E->scenes[slot] = S;
   E->descs[slot] = *desc;
-  E->nl_valid = false;
+  E->nl_valid = E->cflow_valid = false;
   E->steps[slot] = 0;
   if ((r = upload_tables(E))) return r;
   *out_slot = slot;
@@ -956,7 +964,7 @@ int tsph_remove_scene(void* engine, int slot) {
   E->palloc.release(E->scenes[slot].pbase);
   E->calloc_.release(E->scenes[slot].cbase);
   E->scenes[slot].active = 0;
-  E->nl_valid = false;
+  E->nl_valid = E->cflow_valid = false;
   return upload_tables(E);
 }
 
@@ -1060,7 +1068,9 @@ int tsph_set_neighbor_list(void* engine, int enable) {
   tsph_engine* E = (tsph_engine*)engine;
   if (!E) return fail(TSPH_ERR_INVALID_ARG, "null engine");
   HIP_TRY(hipStreamSynchronize(E->stream));
+  if (enable < 0 || enable > 2) return fail(TSPH_ERR_INVALID_ARG, "neighbour-list mode must be 0, 1 or 2");
   E->use_nl = enable != 0;
+  E->nl_lds = enable == 2;
   E->nl_valid = false;
   invalidate_graph(E);
   return TSPH_OK;

@@ -184,7 +184,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_bsum(int32_t* __restrict__ 
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(int32_t* __restrict__ count,
                                                           const int32_t* __restrict__ bsum,
-                                                          int32_t* __restrict__ start, int n) {
+                                                          int32_t* __restrict__ start,
+                                                          uint8_t* __restrict__ cflow, int n) {
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int loc[kScanItems];
   int s = 0;
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(int32_t* __restrict__ 
     if (i < n) {
       start[i] = off;
       count[i] = 0;
+      cflow[i] = 0;
     }
     off += loc[k];
   }
@@ -231,6 +233,7 @@ __global__ __launch_bounds__(256) void k_rank_move(KArgs a, int move_vtmp, int m
   const SceneDev& S = a.scenes[s];
   const int d = S.pbase + (b - a.w.start[S.cbase]) + rank;
   a.alt.cell[d] = key - S.cbase;
+  if (is_flow(a.cur.type[p])) a.w.cflow[key] = 1;  // same value from every writer
   a.alt.pos[d] = a.cur.pos[p];
   a.alt.vel[d] = a.cur.vel[p];
   a.alt.mass[d] = a.cur.mass[p];
@@ -307,7 +310,7 @@ __device__ __forceinline__ void neighbours(const SceneDev& S, const KArgs& a, in
   if (NL) {
     const int n = a.w.nlc[p];
     if (n <= a.nl_cap) {
-      for_list<DIM>(S, a.w.nl, n, a.cap, a.cur.pos, p, C.P.x, C.P.y, C.P.z, fn);
+      for_list<DIM>(S, a.w.nl, n, a.nl_cap, a.cur.pos, p, C.P.x, C.P.y, C.P.z, fn);
       return;
     }
   }
@@ -315,26 +318,223 @@ __device__ __forceinline__ void neighbours(const SceneDev& S, const KArgs& a, in
 }
 
 // Neighbour-list build: one stencil walk per particle per step, recording
-// (j << 5 | stencil code) for every neighbour; boundary particles keep only
+// (j << 6 | stencil code) for every neighbour; boundary particles keep only
 // their flow neighbours (the only ones their passes read).  The candidate
 // loop does nothing else, so the rarely taken append is the only divergent
 // work in it.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_nl_build(KArgs a) {
+__global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   const SceneDev& S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
-  uint32_t* __restrict__ nl = a.w.nl;
   const bool bdy = is_bdy(type[p]);
-  const int cap = a.nl_cap, stride = a.cap;
+  const int cap = a.nl_cap;
+  uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
+  if (bdy && use_cflow) {
+    // A boundary particle only records flow neighbours: none can exist if
+    // no stencil cell holds a flow particle (flags set by the last sort).
+    bool any = false;
+    for (int dx = -1; dx <= 1 && !any; ++dx) {
+      const int nx = C.cx + dx;
+      if (nx < 0 || nx >= S.gn[0]) continue;
+      for (int dy = -1; dy <= 1 && !any; ++dy) {
+        const int ny = C.cy + dy;
+        if (ny < 0 || ny >= S.gn[1]) continue;
+        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+          const int nz = C.cz + dz;
+          if (nz < 0 || nz >= S.gn[2]) continue;
+          if (a.w.cflow[S.cbase + (nx * S.gn[1] + ny) * S.gn[2] + nz]) {
+            any = true;
+            break;
+          }
+        }
+      }
+    }
+    if (!any) {
+      a.w.nlc[p] = 0;
+      return;
+    }
+  }
   int n = 0;
   walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int code) {
     if (bdy && !is_flow(type[j])) return;
-    if (n < cap) nl[(size_t)n * stride + p] = nl_entry(j, code);
+    if (n < cap) li[(size_t)n << 6] = nl_entry(j, code);
     ++n;
   });
+  a.w.nlc[p] = n;
+}
+
+// ------------------------------------------------ LDS-staged list build
+// The same list build with the candidates read from LDS.  One wave per
+// workgroup owns one tile.  Cell keys run (row, fast) with the fast axis z
+// in 3D and y in 2D, so the stencil of every cell in a row segment
+// [fmin, fmax] is 3^(DIM-1) key intervals [row', fmin-1 .. fmax+1]; each is
+// one contiguous particle range.  The wave copies those ranges' positions
+// into LDS once and every lane walks its stencil there, in the order of
+// walk() (same pairs, same order, same values).  A tile whose particles
+// span two rows, or whose window exceeds the LDS budget, walks global
+// memory instead.
+constexpr int kLdsCand = 1280;  // staged candidates per wave (20 KiB)
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_xor(v, d, 64);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_xor(v, d, 64);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+
+template <int DIM, class Fn>
+__device__ __forceinline__ void walk_lds(const SceneDev& S, const int32_t* __restrict__ start,
+                                         const float4* spos, const int* rbase, int32_t shift, int i,
+                                         int cx, int cy, int cz, float oix, float oiy, float oiz,
+                                         Fn&& fn) {
+  const int gy = S.gn[1], gz = S.gn[2];
+#pragma unroll 1
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int nx = cx + dx;
+    if (nx < 0 || nx >= S.gn[0]) continue;
+    const float bx = oix + (float)(cx - nx) * S.g;
+#pragma unroll 1
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int ny = cy + dy;
+      if (ny < 0 || ny >= gy) continue;
+      const float by = oiy + (float)(cy - ny) * S.g;
+      const int32_t rowkey = S.cbase + (nx * gy + ny) * gz;
+      const int lb = rbase[DIM == 3 ? (dx + 1) * 3 + (dy + 1) : (dx + 1)];
+#pragma unroll 1
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+        const int nz = cz + dz;
+        if (DIM == 3 && (nz < 0 || nz >= gz)) continue;
+        const float bz = DIM == 3 ? oiz + (float)(cz - nz) * S.g : 0.0f;
+        const int32_t key = rowkey + nz;
+        const int code = ((dx + 1) << 4) | ((dy + 1) << 2) | (dz + 1);
+        const int32_t jb = start[key] + shift;
+        const int32_t je = start[key + 1] + shift;
+#pragma unroll 4
+        for (int32_t j = jb; j < je; ++j) {
+          const float4 pj = spos[j + lb];
+          const float x = bx - pj.x;
+          const float y = by - pj.y;
+          const float z = DIM == 3 ? bz - pj.z : 0.0f;
+          const float r2 = x * x + y * y + z * z;
+          if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w, code);
+        }
+      }
+    }
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow) {
+  __shared__ float4 spos[kLdsCand];
+  __shared__ int rbase[9];
+  const int w = __builtin_amdgcn_readfirstlane((int)xcd_block());
+  const int2 t = a.tiles[w];
+  if (t.x < 0) return;  // unused tile: the whole workgroup leaves
+  const int s = __builtin_amdgcn_readfirstlane(t.x);
+  const int lane = (int)threadIdx.x;
+  const int p = t.y + lane;
+  const SceneDev& S = a.scenes[s];
+  const bool live = p < S.pbase + S.n;
+  const int8_t* __restrict__ type = a.cur.type;
+  const int cap = a.nl_cap;
+  const int32_t shift = scene_shift(a, S);
+  int cx = 0, cy = 0, cz = 0;
+  float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (live) {
+    decode_cell(S, a.cur.cell[p], cx, cy, cz);
+    P = a.cur.pos[p];
+  }
+  const int gf = DIM == 3 ? S.gn[2] : S.gn[1];
+  const int row = DIM == 3 ? cx * S.gn[1] + cy : cx;
+  const int f = DIM == 3 ? cz : cy;
+  const int rmin = wave_min(live ? row : INT32_MAX), rmax = wave_max(live ? row : INT32_MIN);
+  const int fmin = wave_min(live ? f : INT32_MAX), fmax = wave_max(live ? f : INT32_MIN);
+  // Stencil rows of the segment and their staged particle ranges.
+  int total = 0;
+  bool staged = rmin == rmax;
+  const int fa = fmin - 1 > 0 ? fmin - 1 : 0;
+  const int fb = fmax + 1 < gf - 1 ? fmax + 1 : gf - 1;
+  int jb_k[9], cnt_k[9];
+  const int nrow = DIM == 3 ? 9 : 3;
+  if (staged) {
+    const int rx = DIM == 3 ? rmin / S.gn[1] : rmin;
+    const int ry = DIM == 3 ? rmin - rx * S.gn[1] : 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      jb_k[k] = 0;
+      cnt_k[k] = 0;
+      if (k >= nrow) continue;
+      const int nx = rx + (DIM == 3 ? k / 3 : k) - 1;
+      const int ny = ry + (DIM == 3 ? k % 3 - 1 : 0);
+      if (nx < 0 || nx >= S.gn[0] || ny < 0 || ny >= S.gn[1]) continue;
+      const int32_t rk = S.cbase + (DIM == 3 ? (nx * S.gn[1] + ny) * S.gn[2] : nx * S.gn[1]);
+      jb_k[k] = a.w.start[rk + fa] + shift;
+      cnt_k[k] = a.w.start[rk + fb + 1] + shift - jb_k[k];
+      total += cnt_k[k];
+    }
+    staged = total <= kLdsCand;
+  }
+  if (staged) {
+    int off = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if (k >= nrow) break;
+      for (int q = lane; q < cnt_k[k]; q += 64) spos[off + q] = a.cur.pos[jb_k[k] + q];
+      if (lane == 0) rbase[k] = off - jb_k[k];
+      off += cnt_k[k];
+    }
+    __syncthreads();
+  }
+  if (!live) return;
+  const bool bdy = is_bdy(type[p]);
+  if (bdy && use_cflow) {
+    bool any = false;
+    for (int dx = -1; dx <= 1 && !any; ++dx) {
+      const int nx = cx + dx;
+      if (nx < 0 || nx >= S.gn[0]) continue;
+      for (int dy = -1; dy <= 1 && !any; ++dy) {
+        const int ny = cy + dy;
+        if (ny < 0 || ny >= S.gn[1]) continue;
+        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+          const int nz = cz + dz;
+          if (nz < 0 || nz >= S.gn[2]) continue;
+          if (a.w.cflow[S.cbase + (nx * S.gn[1] + ny) * S.gn[2] + nz]) {
+            any = true;
+            break;
+          }
+        }
+      }
+    }
+    if (!any) {
+      a.w.nlc[p] = 0;
+      return;
+    }
+  }
+  int n = 0;
+  uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
+  auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
+    if (bdy && !is_flow(type[j])) return;
+    if (n < cap) li[(size_t)n << 6] = nl_entry(j, code);
+    ++n;
+  };
+  if (staged) {
+    walk_lds<DIM>(S, a.w.start, spos, rbase, shift, p, cx, cy, cz, P.x, P.y, P.z, rec);
+  } else {
+    walk<DIM>(S, a.w.start, a.cur.pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
+  }
   a.w.nlc[p] = n;
 }
 

@@ -108,9 +108,10 @@ class Server:
         req.admitted = now
         return True
 
-    def run(self, requests, realtime=True):
+    def run(self, requests, realtime=True, log=None):
         """Serve ``requests`` (sorted by arrival).  Returns them with
-        admission / completion times and final globals filled in."""
+        admission / completion times and final globals filled in; ``log``
+        (optional) is called with a line for every retired request."""
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting = {k: [] for k in self.classes}
         t0 = time.perf_counter()
@@ -154,6 +155,9 @@ class Server:
                         c.engine.remove_scene(slot)
                         del c.resident[slot]
                         done.append(r)
+                        if log:
+                            log(f"request {r.rid} ({r.kind}, {r.scene.n} particles, {r.steps} steps) "
+                                f"done at {r.finished:.2f} s")
         self.scene_steps = total_steps
         self.wall = now()
         return done
@@ -218,7 +222,7 @@ def serve_bench(n=24, seed=1, load=0.7, chunk=100, log=None):
     reqs = request_mix(n=n, seed=seed, rate=None)
     cap = plan_capacity(reqs)
     srv = Server(cap, chunk=chunk, max_scenes=n)
-    done = srv.run(reqs, realtime=False)
+    done = srv.run(reqs, realtime=False, log=say)
     t_all = srv.wall
     steps_all = sum(r.steps for r in done)
     capacity = n / t_all
@@ -226,7 +230,7 @@ def serve_bench(n=24, seed=1, load=0.7, chunk=100, log=None):
     rate = load * capacity
     reqs2 = request_mix(n=n, seed=seed, rate=rate)
     srv2 = Server(cap, chunk=chunk, max_scenes=n)
-    done2 = srv2.run(reqs2, realtime=True)
+    done2 = srv2.run(reqs2, realtime=True, log=say)
     lat = [r.latency for r in done2]
     say(f"serve: Poisson stream at {rate:.3f} req/s done in {srv2.wall:.2f} s")
     plain = plain_rate(reqs[: min(n, 12)])
