@@ -117,7 +117,18 @@ def main():
     ap.add_argument("--nl", type=int, default=None, choices=(0, 1, 2),
                     help="fix the neighbour-search mode (default: autotuned with the block sizes)")
     ap.add_argument("--block", type=str, default=None, help="fix pair,elem block sizes, e.g. 256,256")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied 256 MiB x3", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)

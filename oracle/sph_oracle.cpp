@@ -80,7 +80,7 @@ struct Sim {
   int gn[3] = {1, 1, 1};
   int64_t C = 0;
   double grid = 0;
-  R g = 0, rs2 = 0, h1 = 0, kW = 0, kC = 0, eps = (R)1e-8;
+  R g = 0, rs2 = 0, h1 = 0, kW = 0, kC = 0, kG = 0, kGc = 0, eps = (R)1e-8;
   R dt = 0, half_dt = 0, rho0 = 0;
   R visc_c1 = 0, visc_c2 = 0, grav[3] = {0, 0, 0};
   // DP constants (double)
@@ -117,6 +117,9 @@ struct Sim {
     kW = (R)(kw * std::pow(h1d, dim));
     double kcub = dim == 2 ? 15.0 / 7.0 / M_PI : 3.0 / 2.0 / M_PI;
     kC = (R)(kcub * std::pow(h1d, dim));
+    // W'(r)/r factors (q h1 / r = h1^2): -5 kW h1^2 and kC h1^2
+    kG = (R)(-5.0 * kw * std::pow(h1d, dim) * h1d * h1d);
+    kGc = (R)(kcub * std::pow(h1d, dim) * h1d * h1d);
     dt = (R)P.dt;
     half_dt = (R)(0.5 * P.dt);
     rho0 = (R)P.density0;
@@ -164,22 +167,19 @@ struct Sim {
     }
     return R(0);
   }
-  // gradient = fac * x_ij
+  // gradient = fac * x_ij, fac = W'(r)/r (solver_sph_base.py:316-330,
+  // 348-358: k q1^3 (-5 q) h1 r/|r| and k q (1.5 q - 2) h1 r/|r|, with
+  // q h1 / |r| = h1^2 folded into the constants kG, kGc).
   R gradW_fac(R rn) const {
     R q = rn * h1;
     if (!(rn > eps && q <= R(2))) return R(0);
     if (P.kernel == 1) {
       R q1 = R(1) - R(0.5) * q;
-      R s = kW * ((q1 * q1) * q1) * (R(-5) * q) * h1;
-      return s / rn;
+      return kG * ((q1 * q1) * q1);
     }
-    R coef;
-    if (q <= R(1)) coef = kC * q * (R(1.5) * q - R(2));
-    else {
-      R t = R(2) - q;
-      coef = kC * (R(-0.5) * t * t);
-    }
-    return coef * h1 / rn;
+    if (q <= R(1)) return kGc * (R(1.5) * q - R(2));
+    R t = R(2) - q;
+    return kGc * (R(-0.5) * t * t) / q;
   }
 
   // --------------------------------------------------------- neighbour walk
@@ -335,6 +335,7 @@ struct Sim {
   void wc_fluid(int64_t i) {
     R dd = 0, dv[3] = {0, 0, 0};
     const R pri = pr[i] / (rhot[i] * rhot[i]);
+    const R rr = rho0 / rhot[i];
     walk(i, [&](int64_t j, R x, R y, R z, R r2) {
       R rn = std::sqrt(r2);
       R f = gradW_fac(rn);
@@ -347,13 +348,14 @@ struct Sim {
         R vx_ = ux * x + uy * y + uz * z;
         R mn = vx_ < R(0) ? vx_ : R(0);
         R a = visc_c1 * mV[j];
-        if (tj != FLUID) a = a * rho0 / rhot[i];
-        vis = a * mn / (rn * rn + visc_c2);
+        if (tj != FLUID) a = a * rr;
+        vis = a * mn / (r2 + visc_c2);
       }
       R pc = (-rho0 * mV[j]) * (pri + pr[j] / (rhot[j] * rhot[j]));
-      dv[0] += vis * gx + pc * gx;
-      dv[1] += vis * gy + pc * gy;
-      dv[2] += vis * gz + pc * gz;
+      R sc = vis + pc;  // viscosity + pressure force, both along grad W
+      dv[0] += sc * gx;
+      dv[1] += sc * gy;
+      dv[2] += sc * gz;
     });
     drho[i] = dd * rhot[i];
     dvx[i] = dv[0] + grav[0];

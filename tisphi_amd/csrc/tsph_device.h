@@ -40,6 +40,7 @@ struct SceneDev {
   int32_t pbase, n, cbase, ncell;
   double grid, vds[3], dstart[3], dend[3], radius, dt_d;
   float g, rs2, h1, kW, kC, eps;
+  float kG, kGc;  // gradient factors: -5 kW h1^2 (Wendland C2), kC h1^2 (cubic)
   float dt, half_dt, dt6, rho0, visc_c1, visc_c2, grav[3];
   float stiffness, exponent, vsound2, coh, mu, young, h2;
   double alpha, kc, G, K, sin_psi, eps_f;
@@ -122,23 +123,19 @@ __device__ __forceinline__ float kernel_W(const SceneDev& S, float rn) {
   float t = 2.0f - q;
   return S.kC / 6.0f * (t * t * t);
 }
-// grad W = fac * x_ij
+// grad W = fac * x_ij with fac = W'(r)/r.  With q = r h1, q h1 / r = h1^2,
+// so W'(r)/r = -5 kW h1^2 q1^3 (Wendland C2), kC h1^2 (1.5 q - 2) (cubic,
+// q <= 1) and -0.5 kC h1^2 (2 - q)^2 / q (cubic, q > 1).
 __device__ __forceinline__ float kernel_gradfac(const SceneDev& S, float rn) {
   float q = rn * S.h1;
   if (!(rn > S.eps && q <= 2.0f)) return 0.0f;
   if (S.kernel == 1) {
     float q1 = 1.0f - 0.5f * q;
-    float s = S.kW * ((q1 * q1) * q1) * (-5.0f * q) * S.h1;
-    return s / rn;
+    return S.kG * ((q1 * q1) * q1);
   }
-  float coef;
-  if (q <= 1.0f) {
-    coef = S.kC * q * (1.5f * q - 2.0f);
-  } else {
-    float t = 2.0f - q;
-    coef = S.kC * (-0.5f * t * t);
-  }
-  return coef * S.h1 / rn;
+  if (q <= 1.0f) return S.kGc * (1.5f * q - 2.0f);
+  float t = 2.0f - q;
+  return S.kGc * (-0.5f * t * t) / q;
 }
 
 template <class T>

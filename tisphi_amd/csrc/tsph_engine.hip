@@ -280,33 +280,36 @@ struct Seq {
     E->cflow_valid = true;
   }
   // calc_CSPM_f; with neighbour lists on, the walk also records the lists.
-  void shepard() {
+  // skip_fluid: fluid particles get theirs in the step's first fluid pass.
+  void shepard(bool skip_fluid = false) {
     if (E->use_nl) {
       if (E->nl_lds)
         L.go("nl_build", k_nl_build_lds<DIM>, E->tiles_alloc, 64, E->args(), E->cflow_valid ? 1 : 0);
       else
         L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args(), E->cflow_valid ? 1 : 0);
       E->nl_valid = true;
-      L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args());
+      L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
     } else {
-      L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args());
+      L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
     }
   }
   void init_tmp() { L.go("init_tmp", k_init_tmp, tb(), E->elem_block, E->args()); }
   // one_step of the method; with_eos=false when the EOS was fused into the
-  // kernel before (the move of the sort, or lf_half).
-  void one_step(bool with_eos = true) {
-    if (E->nl_valid) one_step_t<true>(with_eos);
-    else one_step_t<false>(with_eos);
+  // kernel before (the move of the sort, or lf_half); shep: the WC fluid
+  // pass also forms the fluid particles' Shepard factors.
+  void one_step(bool with_eos = true, bool shep = false) {
+    if (E->nl_valid) one_step_t<true>(with_eos, shep);
+    else one_step_t<false>(with_eos, shep);
   }
   template <bool NL>
-  void one_step_t(bool with_eos) {
+  void one_step_t(bool with_eos, bool shep) {
     KArgs a = E->args();
     const int B = E->pair_block;
     if (E->method == kWC) {
       if (with_eos) L.go("wc_eos", k_wc_eos, tb(), E->elem_block, a);
       L.go("wc_bdy", k_wc_bdy<DIM, NL>, pb(), B, a);
-      L.go("wc_fluid", k_wc_fluid<DIM, NL>, pb(), B, a);
+      if (shep) L.go("wc_fluid", k_wc_fluid<DIM, NL, true>, pb(), B, a);
+      else L.go("wc_fluid", k_wc_fluid<DIM, NL, false>, pb(), B, a);
     } else if (E->method == kMUI) {
       L.go("mui_soil1", k_mui_soil1<DIM, NL>, pb(), B, a);
       L.go("soil_bdy", k_soil_bdy<DIM, true, NL>, pb(), B, a);
@@ -319,12 +322,12 @@ struct Seq {
   }
   void lf_half(bool eos = false) { L.go("lf_half", k_lf_half, tb(), E->elem_block, E->args(), eos ? 1 : 0); }
   void lf_full() { L.go("lf_full", k_lf_full, tb(), E->elem_block, E->args()); }
-  void substep(int ti) {
+  void substep(int ti, bool shep_first = false) {
     if (ti == 1) {
-      one_step();
+      one_step(true, shep_first);
       lf_full();
     } else if (ti == 2) {
-      one_step();
+      one_step(true, shep_first);
       lf_half();
       one_step();
       lf_full();
@@ -333,7 +336,7 @@ struct Seq {
       L.go("rk", k_rk_init, tb(), E->elem_block, a);
       const int m[4] = {1, 2, 2, 1};
       for (int k = 0; k < 4; ++k) {
-        one_step();
+        one_step(true, shep_first && k == 0);
         L.go("rk", k_rk_update, tb(), E->elem_block, a, m[k]);
         if (k < 3) L.go("rk", k_rk_advect4, tb(), E->elem_block, a);
       }
@@ -377,15 +380,16 @@ struct Seq {
   // the engine's time-integration scheme.
   void step(int ti, bool xsph, bool collide) {
     sort(true);
-    shepard();
+    const bool shep = E->method == kWC;  // fluid Shepard sums in the fluid pass
+    shepard(shep);
     if (ti == 2) {
-      one_step(false);
+      one_step(false, shep);
       lf_half(true);
       one_step(false);
     } else if (ti == 1) {
-      one_step(false);
+      one_step(false, shep);
     } else {
-      substep(4);
+      substep(4, shep);
     }
     const bool lf_in_finish = ti != 4 && !xsph;
     KArgs a = E->args();
@@ -542,6 +546,8 @@ SceneDev make_scene(const tsph_engine* E, const tsph_scene_desc* d) {
   S.kW = (float)(kw * std::pow(h1, d->dim));
   const double kc = d->dim == 2 ? 15.0 / 7.0 / M_PI : 3.0 / 2.0 / M_PI;
   S.kC = (float)(kc * std::pow(h1, d->dim));
+  S.kG = (float)(-5.0 * kw * std::pow(h1, d->dim) * h1 * h1);
+  S.kGc = (float)(kc * std::pow(h1, d->dim) * h1 * h1);
   S.eps = (float)1e-8;
   S.dt = (float)d->dt;
   S.half_dt = (float)(0.5 * d->dt);

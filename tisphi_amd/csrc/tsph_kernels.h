@@ -540,9 +540,10 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow) {
 
 // calc_CSPM_f (solver_sph_base.py:386-398) over the recorded lists.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_shepard_list(KArgs a) {
+__global__ __launch_bounds__(256) void k_shepard_list(KArgs a, int skip_fluid) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
+  if (skip_fluid && a.cur.type[p] == kFluid) return;  // formed by the fluid pass
   const SceneDev& S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
@@ -556,9 +557,10 @@ __global__ __launch_bounds__(256) void k_shepard_list(KArgs a) {
 // =================================================================== Shepard
 // calc_CSPM_f (solver_sph_base.py:386-398)
 template <int DIM>
-__global__ __launch_bounds__(256) void k_shepard(KArgs a) {
+__global__ __launch_bounds__(256) void k_shepard(KArgs a, int skip_fluid) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
+  if (skip_fluid && a.cur.type[p] == kFluid) return;  // formed by the fluid pass
   const SceneDev& S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
@@ -637,8 +639,12 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
 }
 
 // Fluid loop B (solver_sph_wc.py:108-123; tasks solver_sph_base.py:195-203,
-// solver_sph_wc.py:33-71): d_density and d_vel in one walk.
-template <int DIM, bool NL>
+// solver_sph_wc.py:33-71): d_density and d_vel in one walk.  With SHEP
+// the same walk also forms the particle's Shepard sum (calc_CSPM_f,
+// solver_sph_base.py:386-398: flow neighbours, list order), which the fused
+// step then leaves out of the Shepard pass: in WC only boundary particles
+// read CSPM_f within the step, so fluid particles get it here, bitwise equal.
+template <int DIM, bool NL, bool SHEP>
 __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -656,7 +662,8 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   const float4 vi = vtmp[p];
   const float pri = pr2[p];
   const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
-  float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f;
+  const float rr = rho0 / vi.w;  // rho0 / rho_i of the boundary viscosity
+  float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f, shep = 0.f;
   neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     const float rn = sqrtf(r2);
     const float f = kernel_gradfac(S, rn);
@@ -665,21 +672,24 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
     const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
     dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
     const int tj = type[j];
+    if (SHEP && is_flow(tj)) shep += mVj * kernel_W(S, rn);
     float vis = 0.0f;
     if (tj == kFluid || is_bdy(tj) || tj == kRigid) {
       const float vx_ = ux * x + uy * y + uz * z;
       const float mn = vx_ < 0.0f ? vx_ : 0.0f;
       float aa = c1 * mVj;
-      if (tj != kFluid) aa = aa * rho0 / vi.w;
-      vis = aa * mn / (rn * rn + c2);
+      if (tj != kFluid) aa = aa * rr;
+      vis = aa * mn / (r2 + c2);
     }
     const float pc = (-rho0 * mVj) * (pri + pr2[j]);
-    dvx += vis * gx + pc * gx;
-    dvy += vis * gy + pc * gy;
-    dvz += vis * gz + pc * gz;
+    const float sc = vis + pc;
+    dvx += sc * gx;
+    dvy += sc * gy;
+    dvz += sc * gz;
   });
   a.w.drho[p] = dd * vi.w;
   a.w.dv[p] = make_float4(dvx + S.grav[0], dvy + S.grav[1], dvz + S.grav[2], 0.0f);
+  if (SHEP) a.w.cspm[p] = shep != 0.0f ? 1.0f / shep : 1.0f;
 }
 
 // ======================================================================= muI
