@@ -206,7 +206,7 @@ struct Sim {
           for (int64_t j = start[k]; j < start[k + 1]; ++j) {
             if (j == i) continue;
             R x = bx - ox[j], y = by - oy[j], z = bz - oz[j];
-            R r2 = x * x + y * y + z * z;
+            R r2 = std::fma(z, z, std::fma(y, y, x * x));  // same roundings as the engine
             if (r2 < rs2) fn(j, x, y, z, r2);
           }
         }

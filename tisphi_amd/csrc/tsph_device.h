@@ -41,6 +41,7 @@ struct SceneDev {
   double grid, vds[3], dstart[3], dend[3], radius, dt_d;
   float g, rs2, h1, kW, kC, eps;
   float kG, kGc;  // gradient factors: -5 kW h1^2 (Wendland C2), kC h1^2 (cubic)
+  float rs2c;     // stencil-cell culling threshold 1.0001 rs2
   float dt, half_dt, dt6, rho0, visc_c1, visc_c2, grav[3];
   float stiffness, exponent, vsound2, coh, mu, young, h2;
   double alpha, kc, G, K, sin_psi, eps_f;
@@ -195,7 +196,14 @@ __device__ __forceinline__ void decode_cell(const SceneDev& S, int32_t c, int& x
 // after advect_pos, before the next hash).
 // fn(j, x, y, z, r2, m_V_j, code) with code = (dx+1)<<4 | (dy+1)<<2 | (dz+1)
 // the stencil cell of j.
-template <int DIM, class Fn>
+//
+// CULL (only with the centre = i's own cell and offsets in [0, g), i.e.
+// right after the hash): skip stencil cells whose box lies farther than
+// r_s from i.  A candidate there is at least the per-axis gap away
+// (offsets are >= 0 and < g), so its r2 exceeds rs2 by more than the
+// rounding of either sum; the threshold rs2c = 1.0001 rs2 keeps that
+// margin.  Visited pairs and their order are unchanged.
+template <int DIM, bool CULL = false, class Fn>
 __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restrict__ start,
                                      const float4* __restrict__ pos, int32_t shift, int i,
                                      int scx, int scy, int scz, int cix, int ciy, int ciz,
@@ -206,16 +214,22 @@ __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restric
     const int nx = scx + dx;
     if (nx < 0 || nx >= S.gn[0]) continue;
     const float bx = oix + (float)(cix - nx) * S.g;
+    const float ax = dx < 0 ? oix : (dx > 0 ? S.g - oix : 0.0f);
 #pragma unroll 1
     for (int dy = -1; dy <= 1; ++dy) {
       const int ny = scy + dy;
       if (ny < 0 || ny >= gy) continue;
       const float by = oiy + (float)(ciy - ny) * S.g;
+      const float ay = dy < 0 ? oiy : (dy > 0 ? S.g - oiy : 0.0f);
       const int32_t rowkey = S.cbase + (nx * gy + ny) * gz;
 #pragma unroll 1
       for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
         const int nz = scz + dz;
         if (DIM == 3 && (nz < 0 || nz >= gz)) continue;
+        if (CULL) {
+          const float az = DIM == 3 ? (dz < 0 ? oiz : (dz > 0 ? S.g - oiz : 0.0f)) : 0.0f;
+          if (fmaf(az, az, fmaf(ay, ay, ax * ax)) > S.rs2c) continue;
+        }
         const float bz = DIM == 3 ? oiz + (float)(ciz - nz) * S.g : 0.0f;
         const int32_t key = rowkey + nz;
         const int code = ((dx + 1) << 4) | ((dy + 1) << 2) | (dz + 1);
@@ -227,7 +241,7 @@ __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restric
           const float x = bx - pj.x;
           const float y = by - pj.y;
           const float z = DIM == 3 ? bz - pj.z : 0.0f;
-          const float r2 = x * x + y * y + z * z;
+          const float r2 = fmaf(z, z, fmaf(y, y, x * x));
           if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w, code);
         }
       }
@@ -270,7 +284,7 @@ __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __re
     const float x = (cx == 0 ? bx[0] : (cx == 1 ? bx[1] : bx[2])) - pj.x;
     const float y = (cy == 0 ? by[0] : (cy == 1 ? by[1] : by[2])) - pj.y;
     const float z = DIM == 3 ? (cz == 0 ? bz[0] : (cz == 1 ? bz[1] : bz[2])) - pj.z : 0.0f;
-    const float r2 = x * x + y * y + z * z;
+    const float r2 = fmaf(z, z, fmaf(y, y, x * x));
     fn(j, x, y, z, r2, pj.w, code);
   }
 }

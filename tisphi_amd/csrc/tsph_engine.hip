@@ -93,6 +93,7 @@ struct tsph_engine {
   bool nl_lds = false;   // build them from LDS-staged cell windows
   bool nl_valid = false; // lists match the current order and positions
   bool cflow_valid = false;  // per-cell flow flags match the current order
+  bool fresh_sort = false;   // offsets renormalised, no motion since the sort
   int32_t nl_cap = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graph_valid = false;
@@ -278,15 +279,17 @@ struct Seq {
     E->parity ^= 1;
     E->nl_valid = false;
     E->cflow_valid = true;
+    E->fresh_sort = true;
   }
   // calc_CSPM_f; with neighbour lists on, the walk also records the lists.
   // skip_fluid: fluid particles get theirs in the step's first fluid pass.
   void shepard(bool skip_fluid = false) {
     if (E->use_nl) {
+      const int cf = E->cflow_valid ? 1 : 0, cull = E->fresh_sort ? 1 : 0;
       if (E->nl_lds)
-        L.go("nl_build", k_nl_build_lds<DIM>, E->tiles_alloc, 64, E->args(), E->cflow_valid ? 1 : 0);
+        L.go("nl_build", k_nl_build_lds<DIM>, E->tiles_alloc, 64, E->args(), cf, cull);
       else
-        L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args(), E->cflow_valid ? 1 : 0);
+        L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args(), cf, cull);
       E->nl_valid = true;
       L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
     } else {
@@ -344,6 +347,7 @@ struct Seq {
     }
   }
   void advect_pos(bool xsph) {
+    E->fresh_sort = false;
     KArgs a = E->args();
     if (xsph) {
       if (E->nl_valid) L.go("xsph", k_xsph<DIM, true>, pb(), E->pair_block, a);
@@ -363,7 +367,10 @@ struct Seq {
       L.go("dp_adv", k_dp_adv, tb(), E->elem_block, a);
     }
   }
-  void enforce_boundary() { L.go("enforce_boundary", k_enforce_boundary, tb(), E->elem_block, E->args()); }
+  void enforce_boundary() {
+    E->fresh_sort = false;
+    L.go("enforce_boundary", k_enforce_boundary, tb(), E->elem_block, E->args());
+  }
   // SPHBase.step (solver_sph_base.py:41-51) as the pass sequence, one
   // launch per reference kernel.
   void step_passes(int ti, bool xsph, bool collide) {
@@ -399,6 +406,7 @@ struct Seq {
       else L.go("xsph", k_xsph<DIM, false>, pb(), E->pair_block, a);
     }
     L.go("finish", k_finish, tb(), E->elem_block, a, lf_in_finish ? 1 : 0, xsph ? 1 : 0);
+    E->fresh_sort = false;
     if (E->method == kMUI) {
       L.go("mui_regu", k_mui_regu<DIM>, pb(), E->pair_block, a);
       L.go("mui_adv", k_mui_adv2, tb(), E->elem_block, a);
@@ -540,6 +548,7 @@ SceneDev make_scene(const tsph_engine* E, const tsph_scene_desc* d) {
   S.g = (float)d->grid_size;
   const float rs = (float)d->support_radius;
   S.rs2 = rs * rs;
+  S.rs2c = S.rs2 * 1.0001f;
   const double h = d->smoothing_len, h1 = 1.0 / h;
   S.h1 = (float)h1;
   const double kw = d->dim == 2 ? 7.0 / (4.0 * M_PI) : 21.0 / (2.0 * M_PI);
@@ -615,7 +624,7 @@ int h2d(tsph_engine* E, T* dptr, int64_t off, const std::vector<T>& in) {
 
 int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool up) {
   if (!slot_ok(E, slot)) return fail(TSPH_ERR_INVALID_ARG, "bad slot");
-  if (up) E->nl_valid = E->cflow_valid = false;
+  if (up) E->nl_valid = E->cflow_valid = E->fresh_sort = false;
   if (field < 0 || field >= TSPH_F_COUNT) return fail(TSPH_ERR_INVALID_ARG, "bad field id");
   const SceneDev& S = E->scenes[slot];
   const int64_t n = S.n, off = S.pbase;
@@ -956,7 +965,7 @@ int tsph_add_scene(void* engine, const tsph_scene_desc* desc, int* out_slot) {
   HIP_TRY(hipMemsetAsync(E->w.status + 2 * slot, 0, 16, E->stream));
   E->scenes[slot] = S;
   E->descs[slot] = *desc;
-  E->nl_valid = E->cflow_valid = false;
+  E->nl_valid = E->cflow_valid = E->fresh_sort = false;
   E->steps[slot] = 0;
   if ((r = upload_tables(E))) return r;
   *out_slot = slot;
@@ -970,7 +979,7 @@ int tsph_remove_scene(void* engine, int slot) {
   E->palloc.release(E->scenes[slot].pbase);
   E->calloc_.release(E->scenes[slot].cbase);
   E->scenes[slot].active = 0;
-  E->nl_valid = E->cflow_valid = false;
+  E->nl_valid = E->cflow_valid = E->fresh_sort = false;
   return upload_tables(E);
 }
 

@@ -323,7 +323,7 @@ __device__ __forceinline__ void neighbours(const SceneDev& S, const KArgs& a, in
 // loop does nothing else, so the rarely taken append is the only divergent
 // work in it.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow) {
+__global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cull) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   const SceneDev& S = a.scenes[s];
@@ -358,11 +358,16 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow) {
     }
   }
   int n = 0;
-  walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int code) {
+  auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
     if (bdy && !is_flow(type[j])) return;
     if (n < cap) li[(size_t)n << 6] = nl_entry(j, code);
     ++n;
-  });
+  };
+  if (cull)
+    walk<DIM, true>(S, a.w.start, a.cur.pos, C.shift, p, C.cx, C.cy, C.cz, C.cx, C.cy, C.cz, C.P.x,
+                    C.P.y, C.P.z, rec);
+  else
+    walk_c<DIM>(S, a, p, C, rec);
   a.w.nlc[p] = n;
 }
 
@@ -395,7 +400,7 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-template <int DIM, class Fn>
+template <int DIM, bool CULL, class Fn>
 __device__ __forceinline__ void walk_lds(const SceneDev& S, const int32_t* __restrict__ start,
                                          const float4* spos, const int* rbase, int32_t shift, int i,
                                          int cx, int cy, int cz, float oix, float oiy, float oiz,
@@ -406,17 +411,23 @@ __device__ __forceinline__ void walk_lds(const SceneDev& S, const int32_t* __res
     const int nx = cx + dx;
     if (nx < 0 || nx >= S.gn[0]) continue;
     const float bx = oix + (float)(cx - nx) * S.g;
+    const float ax = dx < 0 ? oix : (dx > 0 ? S.g - oix : 0.0f);
 #pragma unroll 1
     for (int dy = -1; dy <= 1; ++dy) {
       const int ny = cy + dy;
       if (ny < 0 || ny >= gy) continue;
       const float by = oiy + (float)(cy - ny) * S.g;
+      const float ay = dy < 0 ? oiy : (dy > 0 ? S.g - oiy : 0.0f);
       const int32_t rowkey = S.cbase + (nx * gy + ny) * gz;
       const int lb = rbase[DIM == 3 ? (dx + 1) * 3 + (dy + 1) : (dx + 1)];
 #pragma unroll 1
       for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
         const int nz = cz + dz;
         if (DIM == 3 && (nz < 0 || nz >= gz)) continue;
+        if (CULL) {  // as in walk()
+          const float az = DIM == 3 ? (dz < 0 ? oiz : (dz > 0 ? S.g - oiz : 0.0f)) : 0.0f;
+          if (fmaf(az, az, fmaf(ay, ay, ax * ax)) > S.rs2c) continue;
+        }
         const float bz = DIM == 3 ? oiz + (float)(cz - nz) * S.g : 0.0f;
         const int32_t key = rowkey + nz;
         const int code = ((dx + 1) << 4) | ((dy + 1) << 2) | (dz + 1);
@@ -428,7 +439,7 @@ __device__ __forceinline__ void walk_lds(const SceneDev& S, const int32_t* __res
           const float x = bx - pj.x;
           const float y = by - pj.y;
           const float z = DIM == 3 ? bz - pj.z : 0.0f;
-          const float r2 = x * x + y * y + z * z;
+          const float r2 = fmaf(z, z, fmaf(y, y, x * x));
           if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w, code);
         }
       }
@@ -437,7 +448,7 @@ __device__ __forceinline__ void walk_lds(const SceneDev& S, const int32_t* __res
 }
 
 template <int DIM>
-__global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow) {
+__global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int cull) {
   __shared__ float4 spos[kLdsCand];
   __shared__ int rbase[9];
   const int w = __builtin_amdgcn_readfirstlane((int)xcd_block());
@@ -530,8 +541,12 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow) {
     if (n < cap) li[(size_t)n << 6] = nl_entry(j, code);
     ++n;
   };
-  if (staged) {
-    walk_lds<DIM>(S, a.w.start, spos, rbase, shift, p, cx, cy, cz, P.x, P.y, P.z, rec);
+  if (staged && cull) {
+    walk_lds<DIM, true>(S, a.w.start, spos, rbase, shift, p, cx, cy, cz, P.x, P.y, P.z, rec);
+  } else if (staged) {
+    walk_lds<DIM, false>(S, a.w.start, spos, rbase, shift, p, cx, cy, cz, P.x, P.y, P.z, rec);
+  } else if (cull) {
+    walk<DIM, true>(S, a.w.start, a.cur.pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
   } else {
     walk<DIM>(S, a.w.start, a.cur.pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
   }
