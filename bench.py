@@ -114,9 +114,11 @@ def main():
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
-    ap.add_argument("--nl", type=int, default=None, choices=(0, 1, 2),
-                    help="fix the neighbour-search mode (default: autotuned with the block sizes)")
-    ap.add_argument("--block", type=str, default=None, help="fix pair,elem block sizes, e.g. 256,256")
+    ap.add_argument("--nl", type=int, default=1, choices=(0, 1, 2),
+                    help="neighbour search: 0 stencil walk per pass, 1 lists (default), 2 LDS-built lists")
+    ap.add_argument("--block", type=str, default="256,256", help="pair,elem threads per block")
+    ap.add_argument("--tune", action="store_true",
+                    help="pick block sizes and list mode with the autotuner (untimed steps) instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -156,14 +158,14 @@ def main():
         eng.step(1)
         eng.sync()
         say(f"step {k + 1} in {1e3 * (time.perf_counter() - t1):.1f} ms")
-    if args.nl is None and args.block is None:
+    if args.tune:
         # launch shape: bitwise-neutral, picked by timing real steps (untimed)
         from tisphi_amd.autotune import tune
         tuned = tune(eng, steps=2, quick=True, use_cache=False, log=say)
         shape = tuned["shape"]
     else:
-        pair, elem = (int(v) for v in (args.block or "256,256").split(","))
-        shape = (pair, elem, 1 if args.nl is None else args.nl)
+        pair, elem = (int(v) for v in args.block.split(","))
+        shape = (pair, elem, args.nl)
         eng.set_block(pair, elem)
         eng.set_neighbor_list(shape[2])
     eng.step(args.warmup)
@@ -210,7 +212,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: lattice dam break generated from the scene JSON",
         "config": {"workload": f"3D WCSPH dam break, {sc.n_flow:,} fluid + {sc.n - sc.n_flow:,} dummy "
-                               f"particles, r=0.0025 m, dt=1e-4, Wendland C2, leap-frog"
+                               f"particles, r={sc.geo.particle_radius:g} m, dt={sc.dt:g} s (CFL 0.2 h/c), "
+                               f"Wendland C2, leap-frog"
                    if args.scene == "dam_break_3d" else args.scene,
                    "particles": sc.n, "fluid_particles": sc.n_flow,
                    "cells": sc.geo.grid_num_total, "parallelism": "single GPU",

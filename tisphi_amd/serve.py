@@ -195,9 +195,11 @@ def percentile(v, q):
 
 def plain_rate(requests, steps=10):
     """Scene-steps/s of the one-scene-per-launch path on the same mix: each
-    request alone in its own engine, stepped and synchronised one after the
-    other (``steps`` steps each, after one warm-up step)."""
-    total_t = 0.0
+    request alone in its own engine, one step launched and synchronised at
+    a time.  Each request's per-step time is measured over ``steps`` steps
+    (after one warm-up step) and charged for its whole step budget, so the
+    rate covers the same work as the served run."""
+    work_s = 0.0
     for r in requests:
         sc = r.scene
         e = _lib.Engine(sc.method, sc.geo.dim, sc.n, sc.geo.grid_num_total, 1)
@@ -209,9 +211,9 @@ def plain_rate(requests, steps=10):
         for _ in range(steps):
             e.step(1)
             e.sync()
-        total_t += time.perf_counter() - t0
+        work_s += (time.perf_counter() - t0) / steps * r.steps
         e.close()
-    return len(requests) * steps / total_t
+    return sum(r.steps for r in requests) / work_s
 
 
 def serve_bench(n=24, seed=1, load=0.7, chunk=100, log=None):
@@ -233,7 +235,7 @@ def serve_bench(n=24, seed=1, load=0.7, chunk=100, log=None):
     done2 = srv2.run(reqs2, realtime=True, log=say)
     lat = [r.latency for r in done2]
     say(f"serve: Poisson stream at {rate:.3f} req/s done in {srv2.wall:.2f} s")
-    plain = plain_rate(reqs[: min(n, 12)])
+    plain = plain_rate(reqs)
     say(f"serve: plain path {plain:.1f} scene-steps/s")
     out = dict(requests=n, arrival_rate_per_s=rate, load=load,
                capacity_requests_per_s=capacity,
