@@ -254,12 +254,20 @@ constexpr int kNlShift = 6;
 __device__ __forceinline__ uint32_t nl_entry(int j, int code) {
   return ((uint32_t)j << kNlShift) | (uint32_t)code;
 }
-// Storage of the lists, blocked per 64-particle tile: entry k of particle p
-// sits at nl_base(p) + 64*k.  A wave's lists are one contiguous run of
-// 64*nl_cap entries whatever the engine capacity, so its list reads and
-// its (lane-divergent) list writes stay within a page or two.
+// Storage of the lists, blocked per 64-particle tile and, inside a tile,
+// in runs of 16 entries per particle: entry k of particle p sits at
+// nl_base(p) + nl_at(k) with nl_base = tile * 64 * nl_cap + lane * 16 and
+// nl_at(k) = (k / 16) * 1024 + k % 16.  A wave's lists are one contiguous
+// run of 64 * nl_cap entries whatever the engine capacity (its traffic
+// stays within a page or two), and each lane appends into its own 64-byte
+// run, so the lane-divergent appends of the build fill whole lines in L2
+// instead of scattering 4-byte writes over 64 lanes' rows.
+// nl_cap is a multiple of 16.
 __device__ __forceinline__ size_t nl_base(int p, int nl_cap) {
-  return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + (size_t)(p & 63);
+  return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + ((size_t)(p & 63) << 4);
+}
+__device__ __forceinline__ size_t nl_at(int k) {
+  return ((size_t)(k >> 4) << 10) + (size_t)(k & 15);
 }
 
 // Iterate a particle's stored neighbour list (built from the same walk, so
@@ -276,7 +284,7 @@ __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __re
   const float bz[3] = {oiz + S.g, oiz, oiz + (-1.0f) * S.g};
 #pragma unroll 2
   for (int k = 0; k < n; ++k) {
-    const uint32_t e = li[(size_t)k << 6];
+    const uint32_t e = li[nl_at(k)];
     const int j = (int)(e >> kNlShift);
     const int code = (int)(e & 63u);
     const float4 pj = pos[j];

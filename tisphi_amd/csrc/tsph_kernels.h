@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
   int n = 0;
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
     if (bdy && !is_flow(type[j])) return;
-    if (n < cap) li[(size_t)n << 6] = nl_entry(j, code);
+    if (n < cap) li[nl_at(n)] = nl_entry(j, code);
     ++n;
   };
   if (cull)
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
   uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
     if (bdy && !is_flow(type[j])) return;
-    if (n < cap) li[(size_t)n << 6] = nl_entry(j, code);
+    if (n < cap) li[nl_at(n)] = nl_entry(j, code);
     ++n;
   };
   if (staged && cull) {
