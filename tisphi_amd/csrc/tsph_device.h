@@ -72,7 +72,8 @@ struct PWork {
   int32_t* tmp;      // scan position -> particle
   int32_t* sid;      // particle -> scene slot (static per scene range)
   float* p;          // pressure
-  float* pr2;        // pressure / density_tmp^2 (WC pressure-force term)
+  float4* vp;        // WC pair record of a particle: v_tmp, p / density_tmp^2;
+                     // sign bit of .w set for a boundary particle
   float* cspm;       // Shepard factor
   float* drho;       // d_density
   float4* dv;        // d_vel
@@ -137,6 +138,43 @@ __device__ __forceinline__ float kernel_gradfac(const SceneDev& S, float rn) {
   if (q <= 1.0f) return S.kGc * (1.5f * q - 2.0f);
   float t = 2.0f - q;
   return S.kGc * (-0.5f * t * t) / q;
+}
+
+// The same two functions with the kernel type fixed at compile time and no
+// branch (both cubic pieces are formed and one is selected): identical
+// values, but a pair loop built on them is straight-line code, so the
+// loads of several neighbours can be in flight together.
+template <int KT>
+__device__ __forceinline__ float kernel_W_t(const SceneDev& S, float rn) {
+  const float q = rn * S.h1;
+  float w;
+  if (KT == 1) {
+    const float q1 = 1.0f - 0.5f * q;
+    const float q2 = q1 * q1;
+    w = S.kW * (q2 * q2) * (1.0f + 2.0f * q);
+  } else {
+    const float qq = q * q;
+    const float wi = S.kC * (0.5f * (qq * q) - qq + (float)(2.0 / 3.0));
+    const float t = 2.0f - q;
+    const float wo = S.kC / 6.0f * (t * t * t);
+    w = q <= 1.0f ? wi : wo;
+  }
+  return (rn > S.eps && q <= 2.0f) ? w : 0.0f;
+}
+template <int KT>
+__device__ __forceinline__ float kernel_gradfac_t(const SceneDev& S, float rn) {
+  const float q = rn * S.h1;
+  float f;
+  if (KT == 1) {
+    const float q1 = 1.0f - 0.5f * q;
+    f = S.kG * ((q1 * q1) * q1);
+  } else {
+    const float fi = S.kGc * (1.5f * q - 2.0f);
+    const float t = 2.0f - q;
+    const float fo = S.kGc * (-0.5f * t * t) / q;
+    f = q <= 1.0f ? fi : fo;
+  }
+  return (rn > S.eps && q <= 2.0f) ? f : 0.0f;
 }
 
 template <class T>
@@ -235,15 +273,23 @@ __device__ __forceinline__ void walk(const SceneDev& S, const int32_t* __restric
         const int code = ((dx + 1) << 4) | ((dy + 1) << 2) | (dz + 1);
         const int32_t jb = start[key] + shift;
         const int32_t je = start[key + 1] + shift;
-#pragma unroll 2
-        for (int32_t j = jb; j < je; ++j) {
-          const float4 pj = pos[j];
+        auto cand = [&](int32_t j, const float4& pj) {
           const float x = bx - pj.x;
           const float y = by - pj.y;
           const float z = DIM == 3 ? bz - pj.z : 0.0f;
           const float r2 = fmaf(z, z, fmaf(y, y, x * x));
           if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w, code);
+        };
+        int32_t j = jb;
+        // four candidates' loads in flight before the first test
+        for (; j + 4 <= je; j += 4) {
+          const float4 p0 = pos[j], p1 = pos[j + 1], p2 = pos[j + 2], p3 = pos[j + 3];
+          cand(j, p0);
+          cand(j + 1, p1);
+          cand(j + 2, p2);
+          cand(j + 3, p3);
         }
+        for (; j < je; ++j) cand(j, pos[j]);
       }
     }
   }
@@ -279,21 +325,39 @@ __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __re
   const uint32_t* __restrict__ li = nl + nl_base(i, nl_cap);
   // shifted centres oi + (c_i - c_j) g for c_j - c_i = -1, 0, +1 (exact
   // products, rounded once: the same values the walk forms)
-  const float bx[3] = {oix + S.g, oix, oix + (-1.0f) * S.g};
-  const float by[3] = {oiy + S.g, oiy, oiy + (-1.0f) * S.g};
-  const float bz[3] = {oiz + S.g, oiz, oiz + (-1.0f) * S.g};
-#pragma unroll 2
-  for (int k = 0; k < n; ++k) {
-    const uint32_t e = li[nl_at(k)];
+  const float bx0 = oix + S.g, bx2 = oix + (-1.0f) * S.g;
+  const float by0 = oiy + S.g, by2 = oiy + (-1.0f) * S.g;
+  const float bz0 = oiz + S.g, bz2 = oiz + (-1.0f) * S.g;
+  auto pick = [](int c, float b0, float b1, float b2) {
+    float b = c == 0 ? b0 : b1;  // two selects, no branch
+    return c == 2 ? b2 : b;
+  };
+  auto pair = [&](uint32_t e, const float4& pj) {
     const int j = (int)(e >> kNlShift);
     const int code = (int)(e & 63u);
-    const float4 pj = pos[j];
     const int cx = code >> 4, cy = (code >> 2) & 3, cz = code & 3;
-    const float x = (cx == 0 ? bx[0] : (cx == 1 ? bx[1] : bx[2])) - pj.x;
-    const float y = (cy == 0 ? by[0] : (cy == 1 ? by[1] : by[2])) - pj.y;
-    const float z = DIM == 3 ? (cz == 0 ? bz[0] : (cz == 1 ? bz[1] : bz[2])) - pj.z : 0.0f;
+    const float x = pick(cx, bx0, oix, bx2) - pj.x;
+    const float y = pick(cy, by0, oiy, by2) - pj.y;
+    const float z = DIM == 3 ? pick(cz, bz0, oiz, bz2) - pj.z : 0.0f;
     const float r2 = fmaf(z, z, fmaf(y, y, x * x));
     fn(j, x, y, z, r2, pj.w, code);
+  };
+  int k = 0;
+  // Four entries at a time: one 16-byte load (a lane's entries are
+  // contiguous within its 16-entry runs), then the four neighbours' loads
+  // together, then the four pairs in list order.
+  for (; k + 4 <= n; k += 4) {
+    const uint4 e4 = *reinterpret_cast<const uint4*>(li + nl_at(k));
+    const float4 p0 = pos[e4.x >> kNlShift], p1 = pos[e4.y >> kNlShift];
+    const float4 p2 = pos[e4.z >> kNlShift], p3 = pos[e4.w >> kNlShift];
+    pair(e4.x, p0);
+    pair(e4.y, p1);
+    pair(e4.z, p2);
+    pair(e4.w, p3);
+  }
+  for (; k < n; ++k) {
+    const uint32_t e = li[nl_at(k)];
+    pair(e, pos[e >> kNlShift]);
   }
 }
 

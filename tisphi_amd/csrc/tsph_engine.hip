@@ -166,7 +166,7 @@ int alloc_work(tsph_engine* E) {
   const size_t nb = (c + kScanTile - 1) / kScanTile;
   int r;
   if ((r = dalloc(&w.key, n, b)) || (r = dalloc(&w.slot, n, b)) || (r = dalloc(&w.tmp, n, b)) ||
-      (r = dalloc(&w.sid, n, b)) || (r = dalloc(&w.p, n, b)) || (r = dalloc(&w.pr2, n, b)) ||
+      (r = dalloc(&w.sid, n, b)) || (r = dalloc(&w.p, n, b)) || (r = dalloc(&w.vp, n, b)) ||
       (r = dalloc(&w.cspm, n, b)) ||
       (r = dalloc(&w.drho, n, b)) || (r = dalloc(&w.dv, n, b)) || (r = dalloc(&w.xs, n, b)) ||
       (r = dalloc(&w.drho_rk, n, b)) || (r = dalloc(&w.dv_rk, n, b)) ||
@@ -190,7 +190,7 @@ int alloc_work(tsph_engine* E) {
 }
 
 void free_work(PWork& w) {
-  void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.pr2, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
+  void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.vp, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
                   w.dsig_d, w.dseq, w.dseqp, w.xs, w.drho_rk, w.dv_rk, w.dsig_rk, w.count,
                   w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc, w.cflow};
   for (void* p : ptrs)
@@ -779,8 +779,18 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
     case TSPH_F_TYPE: {
       std::vector<int8_t> v;
       if (up) {
+        // A scene holds the material of its method and boundary particles
+        // only (WC: fluid; muI / DP: soil); the pair kernels rely on it.
+        const int flow = E->method == kWC ? kFluid : kSoil;
         v.resize(n);
-        for (int64_t i = 0; i < n; ++i) v[i] = (int8_t)hi[i];
+        for (int64_t i = 0; i < n; ++i) {
+          const int t = hi[i];
+          if (t != flow && t != kDummy && t != kRep)
+            return fail(TSPH_ERR_UNSUPPORTED, "particle type " + std::to_string(t) +
+                                                  " in a scene of method " + std::to_string(E->method) +
+                                                  " (only its own material and boundary particles)");
+          v[i] = (int8_t)t;
+        }
         return h2d(E, cur.type, off, v);
       }
       if ((r = d2h(E, (const int8_t*)cur.type, off, n, v))) return r;

@@ -31,7 +31,7 @@ __device__ __forceinline__ bool tile_thread(const KArgs& a, int& s, int& p) {
   if (t.x < 0) return false;
   s = __builtin_amdgcn_readfirstlane(t.x);
   p = t.y + (int)(threadIdx.x & 63);
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   return p < S.pbase + S.n;
 }
 
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_hash(KArgs a) {
 
 template <int DIM>
 __device__ __forceinline__ int32_t hash_one(const KArgs& a, int s, int p) {
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   int cx, cy, cz;
   decode_cell(S, a.cur.cell[p], cx, cy, cz);
   float4 P = a.cur.pos[p];
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void k_rank_move(KArgs a, int move_vtmp, int m
   int rank = 0;
   for (int q = b; q < e; ++q) rank += (a.w.tmp[q] < p) ? 1 : 0;
   const int s = a.w.sid[p];
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int d = S.pbase + (b - a.w.start[S.cbase]) + rank;
   a.alt.cell[d] = key - S.cbase;
   if (is_flow(a.cur.type[p])) a.w.cflow[key] = 1;  // same value from every writer
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_rank_move(KArgs a, int move_vtmp, int m
       if (S.method == kWC && t == kFluid) {
         const float pv = wc_eos(S, v.w);
         a.w.p[d] = pv;
-        a.w.pr2[d] = pv / (v.w * v.w);
+        a.w.vp[d] = make_float4(v.x, v.y, v.z, pv / (v.w * v.w));
       }
     }
     if (t == kSoil && S.method == kMUI) {
@@ -317,6 +317,37 @@ __device__ __forceinline__ void neighbours(const SceneDev& S, const KArgs& a, in
   walk_c<DIM>(S, a, p, C, fn);
 }
 
+// Appends of one particle's list, four entries per 16-byte store (a
+// lane's entries k..k+3, k % 4 == 0, are contiguous and 16-byte aligned):
+// a quarter of the write transactions of one store per entry.
+struct ListAppender {
+  uint32_t* __restrict__ li;
+  int cap, n = 0;
+  uint32_t b0 = 0, b1 = 0, b2 = 0;
+  __device__ __forceinline__ void add(uint32_t e) {
+    const int r = n & 3;
+    if (r == 3) {
+      if (n < cap) *reinterpret_cast<uint4*>(li + nl_at(n - 3)) = make_uint4(b0, b1, b2, e);
+    } else {
+      b0 = r == 0 ? e : b0;
+      b1 = r == 1 ? e : b1;
+      b2 = r == 2 ? e : b2;
+    }
+    ++n;
+  }
+  // the last 1-3 entries
+  __device__ __forceinline__ int finish() {
+    const int r = n & 3;
+    if (n <= cap && r > 0) {
+      const int k = n - r;
+      li[nl_at(k)] = b0;
+      if (r > 1) li[nl_at(k + 1)] = b1;
+      if (r > 2) li[nl_at(k + 2)] = b2;
+    }
+    return n;
+  }
+};
+
 // Neighbour-list build: one stencil walk per particle per step, recording
 // (j << 6 | stencil code) for every neighbour; boundary particles keep only
 // their flow neighbours (the only ones their passes read).  The candidate
@@ -326,7 +357,7 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cull) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   const bool bdy = is_bdy(type[p]);
@@ -357,18 +388,17 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
       return;
     }
   }
-  int n = 0;
+  ListAppender ap{li, cap};
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
     if (bdy && !is_flow(type[j])) return;
-    if (n < cap) li[nl_at(n)] = nl_entry(j, code);
-    ++n;
+    ap.add(nl_entry(j, code));
   };
   if (cull)
     walk<DIM, true>(S, a.w.start, a.cur.pos, C.shift, p, C.cx, C.cy, C.cz, C.cx, C.cy, C.cz, C.P.x,
                     C.P.y, C.P.z, rec);
   else
     walk_c<DIM>(S, a, p, C, rec);
-  a.w.nlc[p] = n;
+  a.w.nlc[p] = ap.finish();
 }
 
 // ------------------------------------------------ LDS-staged list build
@@ -457,7 +487,7 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
   const int s = __builtin_amdgcn_readfirstlane(t.x);
   const int lane = (int)threadIdx.x;
   const int p = t.y + lane;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const bool live = p < S.pbase + S.n;
   const int8_t* __restrict__ type = a.cur.type;
   const int cap = a.nl_cap;
@@ -534,12 +564,11 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
       return;
     }
   }
-  int n = 0;
   uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
+  ListAppender ap{li, cap};
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
     if (bdy && !is_flow(type[j])) return;
-    if (n < cap) li[nl_at(n)] = nl_entry(j, code);
-    ++n;
+    ap.add(nl_entry(j, code));
   };
   if (staged && cull) {
     walk_lds<DIM, true>(S, a.w.start, spos, rbase, shift, p, cx, cy, cz, P.x, P.y, P.z, rec);
@@ -550,7 +579,7 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
   } else {
     walk<DIM>(S, a.w.start, a.cur.pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
   }
-  a.w.nlc[p] = n;
+  a.w.nlc[p] = ap.finish();
 }
 
 // calc_CSPM_f (solver_sph_base.py:386-398) over the recorded lists.
@@ -559,7 +588,7 @@ __global__ __launch_bounds__(256) void k_shepard_list(KArgs a, int skip_fluid) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (skip_fluid && a.cur.type[p] == kFluid) return;  // formed by the fluid pass
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   float sum = 0.0f;
@@ -576,7 +605,7 @@ __global__ __launch_bounds__(256) void k_shepard(KArgs a, int skip_fluid) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (skip_fluid && a.cur.type[p] == kFluid) return;  // formed by the fluid pass
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   float sum = 0.0f;
@@ -590,7 +619,7 @@ __global__ __launch_bounds__(256) void k_shepard(KArgs a, int skip_fluid) {
 __global__ __launch_bounds__(256) void k_init_tmp(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int t = a.cur.type[p];
   if (is_real(t)) a.cur.vtmp[p] = a.cur.vel[p];
   if (t == kSoil) {
@@ -614,10 +643,10 @@ __global__ __launch_bounds__(256) void k_wc_eos(KArgs a) {  // solver_sph_wc.py:
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (a.cur.type[p] == kFluid) {
-    const float rt = a.cur.vtmp[p].w;
-    const float pv = wc_eos(a.scenes[s], rt);
+    const float4 vt = a.cur.vtmp[p];
+    const float pv = wc_eos(a.scenes[s], vt.w);
     a.w.p[p] = pv;
-    a.w.pr2[p] = pv / (rt * rt);
+    a.w.vp[p] = make_float4(vt.x, vt.y, vt.z, pv / (vt.w * vt.w));
   }
 }
 
@@ -627,7 +656,7 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (!is_bdy(a.cur.type[p])) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   const float4* __restrict__ vtmp = a.cur.vtmp;
@@ -645,12 +674,15 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   });
   const float cs = a.w.cspm[p];
   const float4 v = a.cur.vel[p];
-  a.cur.vtmp[p] = make_float4(2.0f * v.x - svx * cs, 2.0f * v.y - svy * cs, 2.0f * v.z - svz * cs,
-                              S.rho0);
+  const float4 vt = make_float4(2.0f * v.x - svx * cs, 2.0f * v.y - svy * cs,
+                                2.0f * v.z - svz * cs, S.rho0);
+  a.cur.vtmp[p] = vt;
   const float pv = sp * cs;
   const float pc = pv > 0.0f ? pv : 0.0f;
   a.w.p[p] = pc;
-  a.w.pr2[p] = pc / (S.rho0 * S.rho0);
+  // sign bit marks the boundary particle for the fluid pass
+  a.w.vp[p] = make_float4(vt.x, vt.y, vt.z,
+                          __int_as_float(__float_as_int(pc / (S.rho0 * S.rho0)) | (int)0x80000000));
 }
 
 // Fluid loop B (solver_sph_wc.py:108-123; tasks solver_sph_base.py:195-203,
@@ -659,6 +691,44 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
 // solver_sph_base.py:386-398: flow neighbours, list order), which the fused
 // step then leaves out of the Shepard pass: in WC only boundary particles
 // read CSPM_f within the step, so fluid particles get it here, bitwise equal.
+template <int DIM, bool NL, bool SHEP, int KT>
+__device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, int p) {
+  const Centre C = centre_of(a, S, p);
+  // A WC scene holds fluid and boundary particles only (checked at upload):
+  // a neighbour's pair record says which one it is, so the pass reads
+  // position + m_V and v_tmp + p/rho_tmp^2 per neighbour and nothing else.
+  const float4* __restrict__ vp = a.w.vp;
+  const float4 vi = a.cur.vtmp[p];
+  const float pri = vp[p].w;
+  const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
+  const float rr = rho0 / vi.w;  // rho0 / rho_i of the boundary viscosity
+  float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f, shep = 0.f;
+  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
+    const float rn = sqrtf(r2);
+    const float f = kernel_gradfac_t<KT>(S, rn);
+    const float gx = f * x, gy = f * y, gz = f * z;
+    const float4 q = vp[j];
+    const bool bj = __float_as_int(q.w) < 0;  // boundary neighbour
+    const float ux = vi.x - q.x, uy = vi.y - q.y, uz = vi.z - q.z;
+    dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
+    if (SHEP) shep += bj ? 0.0f : mVj * kernel_W_t<KT>(S, rn);
+    // viscosity_force: fluid and boundary neighbours (solver_sph_wc.py:33-45)
+    const float vx_ = ux * x + uy * y + uz * z;
+    const float mn = vx_ < 0.0f ? vx_ : 0.0f;
+    const float a0 = c1 * mVj;
+    const float aa = bj ? a0 * rr : a0;
+    const float vis = aa * mn / (r2 + c2);
+    const float pc = (-rho0 * mVj) * (pri + fabsf(q.w));
+    const float sc = vis + pc;
+    dvx += sc * gx;
+    dvy += sc * gy;
+    dvz += sc * gz;
+  });
+  a.w.drho[p] = dd * vi.w;
+  a.w.dv[p] = make_float4(dvx + S.grav[0], dvy + S.grav[1], dvz + S.grav[2], 0.0f);
+  if (SHEP) a.w.cspm[p] = shep != 0.0f ? 1.0f / shep : 1.0f;
+}
+
 template <int DIM, bool NL, bool SHEP>
 __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   int s, p;
@@ -669,42 +739,9 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
     a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-  const SceneDev& S = a.scenes[s];
-  const Centre C = centre_of(a, S, p);
-  const int8_t* __restrict__ type = a.cur.type;
-  const float4* __restrict__ vtmp = a.cur.vtmp;
-  const float* __restrict__ pr2 = a.w.pr2;
-  const float4 vi = vtmp[p];
-  const float pri = pr2[p];
-  const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
-  const float rr = rho0 / vi.w;  // rho0 / rho_i of the boundary viscosity
-  float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f, shep = 0.f;
-  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
-    const float rn = sqrtf(r2);
-    const float f = kernel_gradfac(S, rn);
-    const float gx = f * x, gy = f * y, gz = f * z;
-    const float4 vj = vtmp[j];
-    const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
-    dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
-    const int tj = type[j];
-    if (SHEP && is_flow(tj)) shep += mVj * kernel_W(S, rn);
-    float vis = 0.0f;
-    if (tj == kFluid || is_bdy(tj) || tj == kRigid) {
-      const float vx_ = ux * x + uy * y + uz * z;
-      const float mn = vx_ < 0.0f ? vx_ : 0.0f;
-      float aa = c1 * mVj;
-      if (tj != kFluid) aa = aa * rr;
-      vis = aa * mn / (r2 + c2);
-    }
-    const float pc = (-rho0 * mVj) * (pri + pr2[j]);
-    const float sc = vis + pc;
-    dvx += sc * gx;
-    dvy += sc * gy;
-    dvz += sc * gz;
-  });
-  a.w.drho[p] = dd * vi.w;
-  a.w.dv[p] = make_float4(dvx + S.grav[0], dvy + S.grav[1], dvz + S.grav[2], 0.0f);
-  if (SHEP) a.w.cspm[p] = shep != 0.0f ? 1.0f / shep : 1.0f;
+  const SceneDev S = a.scenes[s];
+  if (S.kernel == 1) wc_fluid_one<DIM, NL, SHEP, 1>(a, S, p);
+  else wc_fluid_one<DIM, NL, SHEP, 0>(a, S, p);
 }
 
 // ======================================================================= muI
@@ -719,7 +756,7 @@ __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
     a.w.dseq[p] = 0.0;
     return;
   }
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float4 vi = vtmp[p];
@@ -777,7 +814,7 @@ __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (!is_bdy(a.cur.type[p])) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   const float4* __restrict__ vtmp = a.cur.vtmp;
@@ -854,7 +891,7 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
     a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float* __restrict__ sgt = a.w.sigt_f;
@@ -877,7 +914,7 @@ __global__ __launch_bounds__(256) void k_mui_adv1(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (a.cur.type[p] != kSoil) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   float4 v = a.cur.vel[p];
   v.w = v.w > S.rho0 ? v.w : S.rho0;
   a.cur.vel[p] = v;
@@ -897,7 +934,7 @@ __global__ __launch_bounds__(256) void k_mui_regu(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (a.cur.type[p] != kSoil) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   int sx = C.cx, sy = C.cy, sz = C.cz;
   float tx = C.P.x, ty = C.P.y, tz = C.P.z;
@@ -975,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_dp_adapt(KArgs a) {  // one_step loop 1
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (a.cur.type[p] != kSoil) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int st = a.stride;
   double t[6];
 #pragma unroll
@@ -1079,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
     a.w.dseqp[p] = 0.0;
     return;
   }
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float* __restrict__ sgt = a.w.sigt_f;
@@ -1125,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_dp_adv(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (a.cur.type[p] != kSoil) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   float4 v = a.cur.vel[p];
   v.w = v.w > S.rho0 ? v.w : S.rho0;
   a.cur.vel[p] = v;
@@ -1147,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_dp_adv(KArgs a) {
 __global__ __launch_bounds__(256) void k_lf_half(KArgs a, int eos) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int t = a.cur.type[p];
   if (is_real(t)) {
     float4 vt = a.cur.vtmp[p];
@@ -1162,7 +1199,7 @@ __global__ __launch_bounds__(256) void k_lf_half(KArgs a, int eos) {
     if (eos && S.method == kWC && t == kFluid) {
       const float pv = wc_eos(S, vt.w);
       a.w.p[p] = pv;
-      a.w.pr2[p] = pv / (vt.w * vt.w);
+      a.w.vp[p] = make_float4(vt.x, vt.y, vt.z, pv / (vt.w * vt.w));
     }
   }
   if (t == kSoil && S.method == kDP) {
@@ -1181,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_lf_half(KArgs a, int eos) {
 __global__ __launch_bounds__(256) void k_lf_full(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int t = a.cur.type[p];
   if (is_real(t)) {
     float4 v = a.cur.vel[p];
@@ -1208,7 +1245,7 @@ __global__ __launch_bounds__(256) void k_lf_full(KArgs a) {
 __global__ __launch_bounds__(256) void k_finish(KArgs a, int do_lf, int use_xsph) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int t = a.cur.type[p];
   const int st = a.stride;
   float4 v = a.cur.vel[p];
@@ -1274,7 +1311,7 @@ __global__ __launch_bounds__(256) void k_xsph(KArgs a) {
   if (!tile_thread(a, s, p)) return;
   const int ti = a.cur.type[p];
   if (!is_real(ti)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   const float4* __restrict__ vel = a.cur.vel;
@@ -1297,7 +1334,7 @@ __global__ __launch_bounds__(256) void k_advect_pos(KArgs a, int use_xsph) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (!is_real(a.cur.type[p])) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const float4 v = a.cur.vel[p];
   float4 P = a.cur.pos[p];
   if (use_xsph && S.xsph) {
@@ -1319,7 +1356,7 @@ __global__ __launch_bounds__(256) void k_chk_density(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (a.cur.type[p] != kFluid) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   float4 v = a.cur.vel[p];
   v.w = v.w > S.rho0 ? v.w : S.rho0;
   a.cur.vel[p] = v;
@@ -1330,7 +1367,7 @@ __global__ __launch_bounds__(256) void k_chk_density(KArgs a) {
 __global__ __launch_bounds__(256) void k_enforce_boundary(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   if (S.boundary != 1 || !is_flow(a.cur.type[p])) return;
   int cc[3];
   decode_cell(S, a.cur.cell[p], cc[0], cc[1], cc[2]);
@@ -1380,7 +1417,7 @@ __global__ __launch_bounds__(256) void k_rk_init(KArgs a) {
 __global__ __launch_bounds__(256) void k_rk_update(KArgs a, int m) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int t = a.cur.type[p];
   const float mf = (float)m;
   if (is_real(t)) {
@@ -1400,7 +1437,7 @@ __global__ __launch_bounds__(256) void k_rk_update(KArgs a, int m) {
 __global__ __launch_bounds__(256) void k_rk_advect4(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int t = a.cur.type[p];
   if (is_real(t)) {
     const float4 v = a.cur.vel[p];
@@ -1425,7 +1462,7 @@ __global__ __launch_bounds__(256) void k_rk_advect4(KArgs a) {
 __global__ __launch_bounds__(256) void k_rk_advect(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int t = a.cur.type[p];
   if (is_real(t)) {
     float4 v = a.cur.vel[p];
@@ -1452,7 +1489,7 @@ __global__ __launch_bounds__(256) void k_neighbor_stats(KArgs a, int32_t* __rest
                                                         long long* __restrict__ s2) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int32_t* __restrict__ id0 = a.cur.id0;
   int n = 0;
@@ -1488,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_globals_tile(KArgs a) {
   const int2 t = a.tiles[w];
   if (t.x < 0) return;
   const int s = t.x;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   const int p = t.y + (int)(threadIdx.x & 63);
   double fr = -INFINITY, pe = 0.0, ke = 0.0, m = 0.0;
   if (p < S.pbase + S.n && is_flow(a.cur.type[p])) {
@@ -1520,7 +1557,7 @@ __global__ __launch_bounds__(256) void k_globals_tile(KArgs a) {
 __global__ __launch_bounds__(256) void k_globals_scene(KArgs a, int nslots) {
   const int s = blockIdx.x;
   if (s >= nslots) return;
-  const SceneDev& S = a.scenes[s];
+  const SceneDev S = a.scenes[s];
   if (!S.active) return;
   double fr = -INFINITY, pe = 0.0, ke = 0.0, m = 0.0;
   for (int k = threadIdx.x; k < S.n_tiles; k += blockDim.x) {
