@@ -301,23 +301,20 @@ __device__ __forceinline__ uint32_t nl_entry(int j, int code) {
   return ((uint32_t)j << kNlShift) | (uint32_t)code;
 }
 // Storage of the lists, blocked per 64-particle tile and, inside a tile,
-// in runs of 16 entries per particle: entry k of particle p sits at
-// nl_base(p) + nl_at(k) with nl_base = tile * 64 * nl_cap + lane * 16 and
-// nl_at(k) = (k / 16) * 1024 + k % 16.  A wave's lists are one contiguous
-// run of 64 * nl_cap entries whatever the engine capacity (its traffic
-// stays within a page or two), and each lane appends into its own 64-byte
-// run, so the lane-divergent appends of the build fill whole lines in L2
-// instead of scattering 4-byte writes over 64 lanes' rows.
-// nl_cap is a multiple of 16.
+// in runs of 4 entries per particle: entry k of particle p sits at
+// nl_base(p) + nl_at(k) with nl_base = tile * 64 * nl_cap + lane * 4 and
+// nl_at(k) = (k / 4) * 256 + k % 4.  A wave's lists are one contiguous run
+// of 64 * nl_cap entries whatever the engine capacity (its traffic stays
+// within a page or two); the build stores each lane's entries four at a
+// time (one aligned 16-byte store), and the replay reads entries k..k+3 of
+// all 64 lanes as one contiguous kilobyte.  nl_cap is a multiple of 4.
 __device__ __forceinline__ size_t nl_base(int p, int nl_cap) {
-  return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + ((size_t)(p & 63) << 4);
+  return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + ((size_t)(p & 63) << 2);
 }
 __device__ __forceinline__ size_t nl_at(int k) {
-  return ((size_t)(k >> 4) << 10) + (size_t)(k & 15);
+  return ((size_t)(k >> 2) << 8) + (size_t)(k & 3);
 }
 
-// Iterate a particle's stored neighbour list (built from the same walk, so
-// the pairs, their order and the pair vectors are identical to walk()).
 template <int DIM, class Fn>
 __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __restrict__ nl,
                                          int n, int nl_cap, const float4* __restrict__ pos, int i,
@@ -343,9 +340,9 @@ __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __re
     fn(j, x, y, z, r2, pj.w, code);
   };
   int k = 0;
-  // Four entries at a time: one 16-byte load (a lane's entries are
-  // contiguous within its 16-entry runs), then the four neighbours' loads
-  // together, then the four pairs in list order.
+  // Four entries at a time: one 16-byte load (a lane's entries k..k+3 are
+  // contiguous), then the four neighbours' loads together, then the four
+  // pairs in list order.
   for (; k + 4 <= n; k += 4) {
     const uint4 e4 = *reinterpret_cast<const uint4*>(li + nl_at(k));
     const float4 p0 = pos[e4.x >> kNlShift], p1 = pos[e4.y >> kNlShift];
