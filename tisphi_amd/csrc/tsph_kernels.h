@@ -348,6 +348,24 @@ struct ListAppender {
   }
 };
 
+// neighbours() with the scene's kernel type as a compile-time tag: fn gets
+// KTag<1> (Wendland C2) or KTag<0> (cubic) first, so its kernel values
+// come from kernel_W_t / kernel_gradfac_t (straight-line code).
+template <int K>
+struct KTag {
+  static constexpr int v = K;
+};
+template <int DIM, bool NL, class Fn>
+__device__ __forceinline__ void neighbours_k(const SceneDev& S, const KArgs& a, int p,
+                                             const Centre& C, Fn&& fn) {
+  if (S.kernel == 1)
+    neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj,
+                                        int code) { fn(KTag<1>{}, j, x, y, z, r2, mVj, code); });
+  else
+    neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj,
+                                        int code) { fn(KTag<0>{}, j, x, y, z, r2, mVj, code); });
+}
+
 // Neighbour-list build: one stencil walk per particle per step, recording
 // (j << 6 | stencil code) for every neighbour; boundary particles keep only
 // their flow neighbours (the only ones their passes read).  The candidate
@@ -592,8 +610,9 @@ __global__ __launch_bounds__(256) void k_shepard_list(KArgs a, int skip_fluid) {
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   float sum = 0.0f;
-  neighbours<DIM, true>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
-    if (is_flow(type[j])) sum += mVj * kernel_W(S, sqrtf(r2));
+  neighbours_k<DIM, true>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                  float mVj, int) {
+    if (is_flow(type[j])) sum += mVj * kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
   });
   a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
 }
@@ -663,9 +682,10 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   const float* __restrict__ pr = a.w.p;
   const float gy = S.grav[1];
   float svx = 0.f, svy = 0.f, svz = 0.f, sp = 0.f;
-  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
+  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                  float mVj, int) {
     if (!is_flow(type[j])) return;
-    const float w = kernel_W(S, sqrtf(r2));
+    const float w = kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
     const float4 vj = vtmp[j];
     svx += mVj * vj.x * w;
     svy += mVj * vj.y * w;
@@ -762,8 +782,9 @@ __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
   const float4 vi = vtmp[p];
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   float dd = 0.f;
-  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
-    const float f = kernel_gradfac(S, sqrtf(r2));
+  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                  float mVj, int) {
+    const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
     const float gw[3] = {f * x, f * y, f * z};
     const float4 vj = vtmp[j];
     const float u[3] = {mVj * (vj.x - vi.x), mVj * (vj.y - vi.y), mVj * (vj.z - vi.z)};
@@ -821,9 +842,10 @@ __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
   const float* __restrict__ sgt = a.w.sigt_f;
   const int st = a.stride;
   float sv[3] = {0.f, 0.f, 0.f}, sr = 0.f, ssg[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
+  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                  float mVj, int) {
     if (!is_flow(type[j])) return;
-    const float w = kernel_W(S, sqrtf(r2));
+    const float w = kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
     const float4 vj = vtmp[j];
     sv[0] += mVj * vj.x * w;
     sv[1] += mVj * vj.y * w;
@@ -902,8 +924,9 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float dv[3] = {0.f, 0.f, 0.f};
-  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
-    const float f = kernel_gradfac(S, sqrtf(r2));
+  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                  float mVj, int) {
+    const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
     soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vtmp[j].w, f, x, y, z, dv);
   });
   a.w.dv[p] = soil_dv_finish(S, dv, vi);
@@ -1128,8 +1151,9 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   float dd = 0.f, dv[3] = {0.f, 0.f, 0.f};
-  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
-    const float f = kernel_gradfac(S, sqrtf(r2));
+  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                  float mVj, int) {
+    const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
     const float gw[3] = {f * x, f * y, f * z};
     const float4 vj = vtmp[j];
     const float u[3] = {mVj * (vj.x - vi.x), mVj * (vj.y - vi.y), mVj * (vj.z - vi.z)};
@@ -1317,9 +1341,10 @@ __global__ __launch_bounds__(256) void k_xsph(KArgs a) {
   const float4* __restrict__ vel = a.cur.vel;
   const float4 vi = vel[p];
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
+  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                  float mVj, int) {
     if (type[j] != ti) return;
-    const float w = kernel_W(S, sqrtf(r2));
+    const float w = kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
     const float4 vj = vel[j];
     sx += mVj * (vj.x - vi.x) * w;
     sy += mVj * (vj.y - vi.y) * w;
