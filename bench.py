@@ -79,6 +79,32 @@ def cpu_baseline(scene, max_steps, budget_s=20.0):
                        f"{dt:.1f} s, {cores} OpenMP threads of {os.cpu_count()} logical CPUs")
 
 
+def granular(steps=500):
+    """The 2D granular column collapse (80,000 soil + 4,818 dummy particles,
+    dt 1e-5) under mu(I) and Drucker-Prager: particle-steps/s of the fused
+    step (hipGraph replay), after 2 warm-up steps."""
+    from tisphi_amd import _lib as L
+    from tisphi_amd.eng.configer_builder import SimConfiger
+    from tisphi_amd.scene import build_scene
+    out = {}
+    for key, name in (("mui", "column_collapse_mui"), ("dp", "column_collapse_dp")):
+        sc = build_scene(SimConfiger(os.path.join(ROOT, "tisphi_amd", "scenes", name + ".json")))
+        e = L.Engine(sc.method, sc.geo.dim, sc.n, sc.geo.grid_num_total, 1)
+        s = e.add_scene(sc.desc())
+        L.upload_scene(e, s, sc)
+        e.step(2)
+        e.sync()
+        t0 = time.perf_counter()
+        e.step(steps)
+        e.sync()
+        t = time.perf_counter() - t0
+        out[key] = dict(particles=sc.n, soil_particles=sc.n_flow, steps=steps, ms_per_step=t * 1e3 / steps,
+                        particle_steps_per_s=sc.n_flow * steps / t)
+        e.close()
+        print(f"bench: granular {key} {t * 1e3 / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
 def batched_vs_plain(n_scenes=256, steps=100, plain_steps=10):
     """256 uneven 2D dam breaks (seed 0): scene-steps/s advanced together in
     one engine (hipGraph replay) and one scene per engine, one after the
@@ -114,6 +140,7 @@ def main():
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--no-granular", action="store_true", help="skip the granular (mu(I), DP) measurement")
     ap.add_argument("--nl", type=int, default=1, choices=(0, 1, 2),
                     help="neighbour search: 0 stencil walk per pass, 1 lists (default), 2 LDS-built lists")
     ap.add_argument("--block", type=str, default="256,256", help="pair,elem threads per block")
@@ -224,6 +251,8 @@ def main():
         "status": {"escaped": status[0], "nan": status[1]},
         "globals_after": {"front": g[0], "potential": g[1], "kinetic": g[2], "mass": g[3]},
     }
+    if not args.no_granular:
+        out["granular"] = granular()
     if not args.no_batch:
         out["batched"] = batched_vs_plain()
     if not args.no_serve:

@@ -326,13 +326,10 @@ struct ListAppender {
   uint32_t b0 = 0, b1 = 0, b2 = 0;
   __device__ __forceinline__ void add(uint32_t e) {
     const int r = n & 3;
-    if (r == 3) {
-      if (n < cap) *reinterpret_cast<uint4*>(li + nl_at(n - 3)) = make_uint4(b0, b1, b2, e);
-    } else {
-      b0 = r == 0 ? e : b0;
-      b1 = r == 1 ? e : b1;
-      b2 = r == 2 ? e : b2;
-    }
+    b0 = r == 0 ? e : b0;
+    b1 = r == 1 ? e : b1;
+    b2 = r == 2 ? e : b2;
+    if (r == 3 && n < cap) *reinterpret_cast<uint4*>(li + nl_at(n - 3)) = make_uint4(b0, b1, b2, e);
     ++n;
   }
   // the last 1-3 entries
