@@ -301,18 +301,20 @@ __device__ __forceinline__ uint32_t nl_entry(int j, int code) {
   return ((uint32_t)j << kNlShift) | (uint32_t)code;
 }
 // Storage of the lists, blocked per 64-particle tile and, inside a tile,
-// in runs of 4 entries per particle: entry k of particle p sits at
-// nl_base(p) + nl_at(k) with nl_base = tile * 64 * nl_cap + lane * 4 and
-// nl_at(k) = (k / 4) * 256 + k % 4.  A wave's lists are one contiguous run
-// of 64 * nl_cap entries whatever the engine capacity (its traffic stays
-// within a page or two); the build stores each lane's entries four at a
-// time (one aligned 16-byte store), and the replay reads entries k..k+3 of
-// all 64 lanes as one contiguous kilobyte.  nl_cap is a multiple of 4.
+// in runs of 16 entries per particle: entry k of particle p sits at
+// nl_base(p) + nl_at(k) with nl_base = tile * 64 * nl_cap + lane * 16 and
+// nl_at(k) = (k / 16) * 1024 + k % 16.  A wave's lists are one contiguous
+// run of 64 * nl_cap entries whatever the engine capacity (its traffic
+// stays within a page or two); each lane appends into its own 64-byte run
+// (four entries per aligned 16-byte store), and the replay loads a lane's
+// entries k..k+3 with one 16-byte load.  (Runs of 4, which make the
+// replay's loads one contiguous kilobyte per wave, measured ~4 % slower per
+// step.)  nl_cap is a multiple of 16.
 __device__ __forceinline__ size_t nl_base(int p, int nl_cap) {
-  return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + ((size_t)(p & 63) << 2);
+  return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + ((size_t)(p & 63) << 4);
 }
 __device__ __forceinline__ size_t nl_at(int k) {
-  return ((size_t)(k >> 2) << 8) + (size_t)(k & 3);
+  return ((size_t)(k >> 4) << 10) + (size_t)(k & 15);
 }
 
 template <int DIM, class Fn>
