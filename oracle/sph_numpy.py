@@ -159,9 +159,23 @@ class NumpySPH:
         fac = np.where(tj == FLUID, 1.0, self.rho0 / self.rhot[i])
         vis = np.where(vis_ok, c1 * self.mV[j] * fac * mn / (r ** 2 + 0.01 * self.h ** 2), 0.0)
         pc = -self.rho0 * self.mV[j] * (self.p[i] / self.rhot[i] ** 2 + self.p[j] / self.rhot[j] ** 2)
-        dv = self._sum(i, np.where(sel, vis + pc, 0.0)[:, None] * gw)
+        dv = self._sum(i, np.where(sel, vis + pc, 0.0)[:, None] * gw) + self._rep(i, j, xij, r)
         self.drho[fl] = dd[fl] * self.rhot[fl]
         self.dv[fl] = dv[fl] + self.g
+
+    def _rep(self, i, j, xij, r):
+        """Repulsive force on each particle from its repulsive neighbours
+        (solver_sph_base.py:676-690; boundary 3 / 4, WC and muI only)."""
+        if self.d["boundary"] not in (3, 4) or self.d["method"] == 3:
+            return np.zeros((self.n, 3))
+        dd = self.d["particle_diameter"]
+        chi = np.where((r > 0) & (r < dd), 1.0 - r / dd, 0.0)
+        gam = r / (0.75 * self.h)
+        f = np.where((gam > 0) & (gam <= 2 / 3), 2 / 3,
+                     np.where((gam > 2 / 3) & (gam <= 1), 2 * gam - 1.5 * gam ** 2,
+                              np.where((gam > 1) & (gam < 2), 0.5 * (2 - gam) ** 2, 0.0)))
+        fac = np.where(self.t[j] == REP, 0.01 * self.d["vsound"] ** 2 * chi * f / r ** 2, 0.0)
+        return self._sum(i, fac[:, None] * xij)
 
     def _vgrad(self, i, j, xij, r, sel):
         gw = self.gradW(xij, r)
@@ -191,7 +205,7 @@ class NumpySPH:
         sel = self.t[i] == SOIL
         M = self.sigt[j] / (self.rhot[j] ** 2)[:, None, None] + self.sigt[i] / (self.rhot[i] ** 2)[:, None, None]
         tmp = (self.mV[j] * self.rhot[j])[:, None] * np.einsum("nab,nb->na", M, gw)
-        dv = self._sum(i, np.where(sel[:, None], tmp, 0.0))
+        dv = self._sum(i, np.where(sel[:, None], tmp, 0.0)) + self._rep(i, j, xij, r)
         damp = -5e-5 * np.sqrt(self.d["young"] / (self.rhot * self.h ** 2))
         self.dv[soil] = dv[soil] + self.g + damp[soil, None] * self.vt[soil]
 

@@ -83,6 +83,9 @@ struct Sim {
   R g = 0, rs2 = 0, h1 = 0, kW = 0, kC = 0, kG = 0, kGc = 0, eps = (R)1e-8;
   R dt = 0, half_dt = 0, rho0 = 0;
   R visc_c1 = 0, visc_c2 = 0, grav[3] = {0, 0, 0};
+  // repulsive-particle force (boundary 3 / 4; WC and muI)
+  bool rep = false;
+  R rep_k = 0, rep_d = 0, rep_h = 0;
   // DP constants (double)
   double alpha = 0, kc = 0, G = 0, K = 0, sin_psi = 0, eps_f = 1e-4;
   int64_t escaped = 0;
@@ -126,6 +129,10 @@ struct Sim {
     visc_c1 = (R)(2.0 * (dim + 2) * P.viscosity);
     visc_c2 = (R)(0.01 * h * h);
     for (int a = 0; a < 3; ++a) grav[a] = (R)P.gravity[a];
+    rep = (P.boundary == 3 || P.boundary == 4) && P.method != 3;  // DP leaves it out
+    rep_k = (R)(0.01 * P.vsound * P.vsound);
+    rep_d = (R)P.particle_diameter;
+    rep_h = (R)(0.75 * P.smoothing_len);
     double t = std::tan(P.friction);
     alpha = t / std::sqrt(9 + 12 * t * t);
     kc = 3 * P.cohesion / std::sqrt(9 + 12 * t * t);
@@ -307,6 +314,32 @@ struct Sim {
     }
   }
 
+  // calc_repulsive_force (solver_sph_base.py:676-690): F = fac * x_ij.
+  R rep_fac(R rn) const {
+    R chi = (rn > R(0) && rn < rep_d) ? R(1) - rn / rep_d : R(0);
+    R gam = rn / rep_h;
+    const R two3 = (R)(2.0 / 3.0);
+    R f = 0;
+    if (gam > R(0) && gam <= two3) f = two3;
+    else if (gam > two3 && gam <= R(1)) f = R(2) * gam - R(1.5) * (gam * gam);
+    else if (gam > R(1) && gam < R(2)) {
+      R t = R(2) - gam;
+      f = R(0.5) * (t * t);
+    }
+    return rep_k * chi * f / (rn * rn);
+  }
+  // calc_rep_force_task (solver_sph_wc.py:74-76, solver_sph_muI.py:34-36):
+  // a second sweep after the pair forces, repulsive neighbours only.
+  void add_rep_force(int64_t i, R dv[3]) const {
+    walk(i, [&](int64_t j, R x, R y, R z, R r2) {
+      if (type[j] != REP) return;
+      R f = rep_fac(std::sqrt(r2));
+      dv[0] += f * x;
+      dv[1] += f * y;
+      dv[2] += f * z;
+    });
+  }
+
   // --------------------------------------------------------------- WCSPH
   R eos(R rt) const {  // solver_sph_wc.py:88
     R v = (R)P.stiffness * (ipow_or_pow(rt / rho0, P.exponent) - R(1));
@@ -357,6 +390,7 @@ struct Sim {
       dv[1] += sc * gy;
       dv[2] += sc * gz;
     });
+    if (rep) add_rep_force(i, dv);
     drho[i] = dd * rhot[i];
     dvx[i] = dv[0] + grav[0];
     dvy[i] = dv[1] + grav[1];
@@ -485,6 +519,7 @@ struct Sim {
       if (type[i] == SOIL) {
         R dv[3];
         soil_dvel_sum(i, dv);
+        if (rep) add_rep_force(i, dv);
         soil_dvel_finish(i, dv);
       }
   }

@@ -34,6 +34,9 @@ CASES = {
                                     "domainEnd": [0.6, 0.3, 0.1]}, 20, 0.2),
     "wc2d_se": ("dam_break_2d", {"timeIntegration": 1}, 100, 0.6),
     "wc2d_rk4_collision": ("dam_break_2d", {"timeIntegration": 4, "boundary": 1}, 60, 0.6),
+    "wc2d_rep": ("dam_break_2d", {"boundary": 3}, 100, 0.6),
+    "wc2d_dummy_rep": ("dam_break_2d", {"boundary": 4}, 100, 0.6),
+    "mui2d_dummy_rep": ("column_collapse_mui", {"boundary": 4}, 30, 0.2),
 }
 
 STATE = ["cell", "offset", "vel", "density", "mass", "m_V", "type", "id0", "obj_id", "pressure",

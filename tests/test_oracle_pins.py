@@ -441,10 +441,12 @@ def test_dp_return_mapping_postconditions():
 
 
 # ------------------------------------------------ numpy restatement vs C++
-@pytest.mark.parametrize("method,dim,xsph", [(1, 2, False), (1, 3, False), (2, 2, True), (3, 2, True),
-                                             (3, 3, False), (2, 3, False), (1, 2, True)])
-def test_oracle_matches_numpy_restatement(method, dim, xsph):
-    sc = build_scene(tiny_cfg(method=method, dim=dim, xsph=xsph))
+@pytest.mark.parametrize("method,dim,xsph,boundary", [
+    (1, 2, False, 2), (1, 3, False, 2), (2, 2, True, 2), (3, 2, True, 2), (3, 3, False, 2),
+    (2, 3, False, 2), (1, 2, True, 2), (1, 2, False, 3), (1, 2, False, 4), (2, 2, False, 4),
+    (3, 2, False, 4), (1, 3, False, 4)])
+def test_oracle_matches_numpy_restatement(method, dim, xsph, boundary):
+    sc = build_scene(tiny_cfg(method=method, dim=dim, xsph=xsph, boundary=boundary))
     d = sc.desc()
     o = O.Oracle(d, 64)
     O.load_scene(o, sc)
@@ -469,3 +471,47 @@ def test_oracle_matches_numpy_restatement(method, dim, xsph):
         close(o.get("strain_equ"), ref.seq[idx])
     if method == 3:
         np.testing.assert_array_equal(o.get("flag_retmap"), ref.flag[idx])
+
+
+# ------------------------------------------------ repulsive particles
+def test_repulsive_force_closed_form():
+    """calc_repulsive_force (solver_sph_base.py:676-690) against values
+    worked by hand in the two branches of f(gamma) that act within one
+    particle diameter when kh = 1.5 (gamma <= 2/3 and 2/3 < gamma <= 1);
+    zero beyond one diameter; directed along x_ij; repulsive neighbours
+    only."""
+    d, h, c = 0.01, 0.015, 60.0
+    desc = dict(dim=2, boundary=3, method=1, particle_diameter=d, vsound=c, gravity=[0, -9.81, 0],
+                smoothing_len=h, support_radius=2 * h, dt=1e-4, density0=1000.0, friction=0.0,
+                cohesion=0.0, young=1.0, poisson=0.3, dilatancy=0.0)
+    ref = NumpySPH(desc, np.zeros((2, 3)), np.zeros((2, 3)), np.zeros(2), np.zeros(2), np.zeros(2),
+                   np.array([1, -2]))
+    k = 0.01 * c * c
+    for r, f in ((0.004, 2 / 3),                                   # gamma = 0.356
+                 (0.0095, 2 * (0.0095 / 0.01125) - 1.5 * (0.0095 / 0.01125) ** 2)):  # 0.844
+        chi = 1 - r / d
+        x = np.array([[0.0, r, 0.0]])
+        F = ref._rep(np.array([0]), np.array([1]), x, np.array([r]))
+        np.testing.assert_allclose(F[0], [0.0, k * chi * f / r ** 2 * r, 0.0], rtol=1e-14)
+    # beyond one diameter the force vanishes although gamma < 2
+    x = np.array([[0.0, 0.0105, 0.0]])
+    F = ref._rep(np.array([0]), np.array([1]), x, np.array([0.0105]))
+    assert np.all(F == 0.0)
+    # only repulsive neighbours push
+    ref.t[1] = -1
+    x = np.array([[0.0, 0.004, 0.0]])
+    assert np.all(ref._rep(np.array([0]), np.array([1]), x, np.array([0.004])) == 0.0)
+
+
+def test_repulsive_layer_count_and_placement():
+    """One layer at spacing r on the left, bottom and right walls
+    (particle_func.py:346-374): 2D dam break, r = 0.005: 319 + 645 + 319."""
+    cfg = scene_cfg("dam_break_2d")
+    cfg.config["Configuration"]["boundary"] = 3
+    sc = build_scene(cfg)
+    rep = sc.pos[sc.mat_type == -2]
+    assert sc.n_rep == len(rep) == 319 + 645 + 319
+    assert sc.n == 7200 + sc.n_rep
+    on_wall = (np.isclose(rep[:, 0], 0.0) | np.isclose(rep[:, 0], 3.22) | np.isclose(rep[:, 1], 0.0))
+    assert np.all(on_wall)
+    assert np.allclose(np.diff(np.sort(rep[np.isclose(rep[:, 0], 0.0), 1])), 0.005)

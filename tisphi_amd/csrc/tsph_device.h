@@ -42,6 +42,8 @@ struct SceneDev {
   float g, rs2, h1, kW, kC, eps;
   float kG, kGc;  // gradient factors: -5 kW h1^2 (Wendland C2), kC h1^2 (cubic)
   float rs2c;     // stencil-cell culling threshold 1.0001 rs2
+  int rep;        // add the repulsive-particle force (boundary 3 / 4; WC, muI)
+  float rep_k, rep_d, rep_h;  // 0.01 c^2, particle diameter, 0.75 h
   float dt, half_dt, dt6, rho0, visc_c1, visc_c2, grav[3];
   float stiffness, exponent, vsound2, coh, mu, young, h2;
   double alpha, kc, G, K, sin_psi, eps_f;
@@ -175,6 +177,24 @@ __device__ __forceinline__ float kernel_gradfac_t(const SceneDev& S, float rn) {
     f = q <= 1.0f ? fi : fo;
   }
   return (rn > S.eps && q <= 2.0f) ? f : 0.0f;
+}
+
+// Repulsive-particle force calc_repulsive_force (solver_sph_base.py:676-690):
+// F = fac * x_ij with fac = 0.01 c^2 chi(r) f(r / (0.75 h)) / r^2.
+__device__ __forceinline__ float rep_fac(const SceneDev& S, float rn) {
+  const float chi = (rn > 0.0f && rn < S.rep_d) ? 1.0f - rn / S.rep_d : 0.0f;
+  const float gam = rn / S.rep_h;
+  const float two3 = (float)(2.0 / 3.0);
+  float f = 0.0f;
+  if (gam > 0.0f && gam <= two3) {
+    f = two3;
+  } else if (gam > two3 && gam <= 1.0f) {
+    f = 2.0f * gam - 1.5f * (gam * gam);
+  } else if (gam > 1.0f && gam < 2.0f) {
+    const float t = 2.0f - gam;
+    f = 0.5f * (t * t);
+  }
+  return S.rep_k * chi * f / (rn * rn);
 }
 
 template <class T>

@@ -504,8 +504,7 @@ int validate_desc(const tsph_engine* E, const tsph_scene_desc* d) {
   if (!d) return fail(TSPH_ERR_INVALID_ARG, "null scene description");
   if (d->method != E->method) return fail(TSPH_ERR_INVALID_ARG, "scene method differs from the engine's");
   if (d->dim != E->dim) return fail(TSPH_ERR_INVALID_ARG, "scene dimension differs from the engine's");
-  if (d->boundary < 0 || d->boundary > 2)
-    return fail(TSPH_ERR_UNSUPPORTED, "boundary 3/4 (repulsive particles) is not built");
+  if (d->boundary < 0 || d->boundary > 4) return fail(TSPH_ERR_INVALID_ARG, "boundary must be 0..4");
   if (d->kernel != 0 && d->kernel != 1) return fail(TSPH_ERR_INVALID_ARG, "kernel must be 0 or 1");
   if (d->kernel_corr != 0) return fail(TSPH_ERR_UNSUPPORTED, "kernelCorrection != 0 is not built");
   if (d->time_integration != 1 && d->time_integration != 2 && d->time_integration != 4)
@@ -549,6 +548,12 @@ SceneDev make_scene(const tsph_engine* E, const tsph_scene_desc* d) {
   const float rs = (float)d->support_radius;
   S.rs2 = rs * rs;
   S.rs2c = S.rs2 * 1.0001f;
+  // repulsive layer (boundary 3 / 4): WC and muI add its force; the DP
+  // solver does not (solver_sph_dp.py:262-264 leaves it out)
+  S.rep = (d->boundary == 3 || d->boundary == 4) && d->method != kDP;
+  S.rep_k = (float)(0.01 * d->vsound * d->vsound);
+  S.rep_d = (float)d->particle_diameter;
+  S.rep_h = (float)(0.75 * d->smoothing_len);
   const double h = d->smoothing_len, h1 = 1.0 / h;
   S.h1 = (float)h1;
   const double kw = d->dim == 2 ? 7.0 / (4.0 * M_PI) : 21.0 / (2.0 * M_PI);

@@ -741,6 +741,18 @@ __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, 
     dvy += sc * gy;
     dvz += sc * gz;
   });
+  if (S.rep) {
+    // calc_rep_force_task (solver_sph_wc.py:74-76, 118-121): a second sweep
+    // adding the repulsive particles' force after the pair forces
+    const int8_t* __restrict__ type = a.cur.type;
+    neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float, int) {
+      if (type[j] != kRep) return;
+      const float f = rep_fac(S, sqrtf(r2));
+      dvx += f * x;
+      dvy += f * y;
+      dvz += f * z;
+    });
+  }
   a.w.drho[p] = dd * vi.w;
   a.w.dv[p] = make_float4(dvx + S.grav[0], dvy + S.grav[1], dvz + S.grav[2], 0.0f);
   if (SHEP) a.w.cspm[p] = shep != 0.0f ? 1.0f / shep : 1.0f;
@@ -926,6 +938,18 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
     const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
     soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vtmp[j].w, f, x, y, z, dv);
   });
+  if (S.rep) {
+    // calc_rep_force_task (solver_sph_muI.py:34-36, 121-123), after the
+    // stress forces and before gravity and damping
+    const int8_t* __restrict__ type = a.cur.type;
+    neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float, int) {
+      if (type[j] != kRep) return;
+      const float f = rep_fac(S, sqrtf(r2));
+      dv[0] += f * x;
+      dv[1] += f * y;
+      dv[2] += f * z;
+    });
+  }
   a.w.dv[p] = soil_dv_finish(S, dv, vi);
 }
 

@@ -76,6 +76,27 @@ def calc_dummy_boundary(dim, ds, de, vds, vde):
     ]
 
 
+def calc_rep_boundary(dim, ds, de, r):
+    """Wall boxes [lower, upper] of the repulsive-particle layer: one layer
+    of particles at spacing r on the walls, the top left open
+    (particle_func.py:346-374)."""
+    A = np.array
+    t = r / 2
+    if dim == 3:
+        return [
+            [A([ds[0] - t, ds[1] + t, ds[2] - t]), A([ds[0] + t, de[1] - t, de[2] - t])],
+            [A([ds[0] - t, ds[1] + t, de[2] - t]), A([de[0] - t, de[1] - t, de[2] + t])],
+            [A([de[0] - t, ds[1] + t, ds[2] + t]), A([de[0] + t, de[1] - t, de[2] + t])],
+            [A([ds[0] + t, ds[1] + t, ds[2] - t]), A([de[0] + t, de[1] - t, ds[2] + t])],
+            [A(ds, dtype=np.float64) - t, A([de[0], ds[1], de[2]]) + t],
+        ]
+    return [
+        [A([ds[0] - t, ds[1] + t, ds[2]]), A([ds[0] + t, de[1] - t, de[2]])],
+        [A([ds[0] - t, ds[1] - t, ds[2]]), A([de[0] + t, ds[1] + t, de[2]])],
+        [A([de[0] - t, ds[1] + t, ds[2]]), A([de[0] + t, de[1] - t, de[2]])],
+    ]
+
+
 def count_boundary(boundary, dim, offset):
     return sum(count_cube_num(b[0], b[1], dim, offset) for b in boundary)
 

@@ -7,8 +7,10 @@ constructors, done on the host in float64:
   r_s = kappa*h, grid = ceil(kappa*kh)*d, virtual domain = domain +- grid
   (2D: z extent d and grid_num[2] = 1), grid_num = ceil(vsize/grid).
 * particles (A2)  — eng/particle_system.py:72-108, 135-174, 274-314:
-  blocks in file order, then dummy walls; m_V = d^dim, mass = m_V*rho with
-  rho = 0 for dummies; id0 = generation index (set_id0, :208-211).
+  blocks in file order, then dummy walls, then the repulsive layer
+  (boundary 3 / 4: spacing r, particle_func.py:346-374); m_V = d^dim,
+  mass = m_V*rho with rho = 0 for boundary particles; id0 = generation
+  index (set_id0, :208-211).
 * dt        (A16) — eng/solver_sph_base.py:209-212 with the solvers' sound
   speeds (solver_sph_wc.py:17-21, solver_sph_muI.py:23-25,
   solver_sph_dp.py:20-22).
@@ -20,8 +22,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .eng.particle_func import (
-    MAT_DUMMY, MAT_FLUID, MAT_RIGID, MAT_SOIL, calc_cube_particle_num,
-    calc_dummy_boundary, chk_block_in_domain, count_boundary, cube_positions,
+    MAT_DUMMY, MAT_FLUID, MAT_REP, MAT_RIGID, MAT_SOIL, calc_cube_particle_num,
+    calc_dummy_boundary, calc_rep_boundary, chk_block_in_domain, count_boundary, cube_positions,
     get_material, set_material)
 
 METHOD_WC, METHOD_MUI, METHOD_DP = 1, 2, 3
@@ -103,6 +105,7 @@ class Scene:
     mat_rigid: list
     block_counts: list = field(default_factory=list)
     n_dummy: int = 0
+    n_rep: int = 0
     # host arrays, generation order
     pos: np.ndarray = None
     vel: np.ndarray = None
@@ -117,7 +120,7 @@ class Scene:
     @property
     def n(self):
         if self.pos is None:
-            return int(sum(self.block_counts) + self.n_dummy)
+            return int(sum(self.block_counts) + self.n_dummy + self.n_rep)
         return int(self.pos.shape[0])
 
     @property
@@ -157,8 +160,8 @@ def build_scene(cfg, generate=True) -> Scene:
     if method not in (METHOD_WC, METHOD_MUI, METHOD_DP):
         raise NotImplementedError(f"Solver type {method} has not been implemented.")
     boundary = cfg.get_cfg("boundary")
-    if boundary in (BDY_REP, BDY_DUMMY_REP):
-        raise NotImplementedError("repulsive-particle boundaries (3, 4) are not built")
+    if boundary not in (BDY_NONE, BDY_COLLISION, BDY_DUMMY, BDY_REP, BDY_DUMMY_REP):
+        raise NotImplementedError(f"boundary {boundary} is not built")
     kernel = cfg.get_cfg("kernel")
     kcorr = cfg.get_cfg("kernelCorrection")
     if kcorr != 0:
@@ -200,6 +203,11 @@ def build_scene(cfg, generate=True) -> Scene:
         dummy = calc_dummy_boundary(geo.dim, geo.domain_start, geo.domain_end,
                                     geo.vdomain_start, geo.vdomain_end)
         sc.n_dummy = count_boundary(dummy, geo.dim, geo.particle_diameter)
+    rep = None
+    if boundary in (BDY_REP, BDY_DUMMY_REP):
+        # one layer at spacing r on the walls (particle_system.py:101-104, 147-148)
+        rep = calc_rep_boundary(geo.dim, geo.domain_start, geo.domain_end, geo.particle_radius)
+        sc.n_rep = count_boundary(rep, geo.dim, geo.particle_radius)
     if not generate:
         return sc
 
@@ -225,6 +233,16 @@ def build_scene(cfg, generate=True) -> Scene:
             rho.append(np.zeros(n))
             mtype.append(np.full(n, MAT_DUMMY, dtype=np.int32))
             oid.append(np.full(n, MAT_DUMMY, dtype=np.int32))
+            mid.append(np.zeros(n, dtype=np.int32))
+    if rep is not None:
+        for lo, hi in rep:
+            p = cube_positions(lo, hi - lo, geo.dim, geo.particle_radius)
+            n = p.shape[0]
+            pos.append(p)
+            vel.append(np.zeros((n, 3)))
+            rho.append(np.zeros(n))
+            mtype.append(np.full(n, MAT_REP, dtype=np.int32))
+            oid.append(np.full(n, MAT_REP, dtype=np.int32))
             mid.append(np.zeros(n, dtype=np.int32))
     sc.pos = np.concatenate(pos) if pos else np.zeros((0, 3))
     sc.vel = np.concatenate(vel) if vel else np.zeros((0, 3))
