@@ -134,7 +134,11 @@ def test_scene_json_sections_and_errors(tmp_path):
     with pytest.raises(NotImplementedError):
         build_scene(SimConfiger(config=cfg2))
     cfg3 = json.loads(json.dumps(cfg))
-    cfg3["Configuration"]["boundary"] = 3
+    cfg3["Configuration"]["boundary"] = 5
+    with pytest.raises(NotImplementedError):
+        build_scene(SimConfiger(config=cfg3))
+    cfg3["Configuration"]["kernelCorrection"] = 2
+    cfg3["Configuration"]["boundary"] = 2
     with pytest.raises(NotImplementedError):
         build_scene(SimConfiger(config=cfg3))
     cfg4 = json.loads(json.dumps(cfg))
