@@ -123,6 +123,33 @@ class NumpySPH:
         i, j, xij, r = self.pairs()
         s = self._sum(i, np.where(is_flow(self.t[j]), self.mV[j] * self.W(r), 0.0))
         self.cspm = np.where(s != 0.0, 1.0 / np.where(s != 0, s, 1.0), 1.0)
+        if self.d.get("kernel_corr", 0) == 1:
+            self.cspm_L(i, j, xij, r)
+
+    def cspm_L(self, i, j, xij, r):
+        """calc_CSPM_L (solver_sph_base.py:400-423), inverted (np.linalg)."""
+        gw = self.gradW(xij, r)
+        same = is_flow(self.t[i]) & (self.t[i] == self.t[j])
+        T = np.where(same[:, None, None], (self.mV[j][:, None] * -xij)[:, :, None] * gw[:, None, :], 0.0)
+        L = self._sum(i, T)
+        M = np.tile(np.eye(3), (self.n, 1, 1))
+        flow = is_flow(self.t)
+        if self.dim == 2:
+            d = np.linalg.det(L[:, :2, :2])
+            ok = flow & (np.abs(d) > 1e-8)
+            M[ok] = 0.0
+            M[ok, :2, :2] = np.linalg.inv(L[ok, :2, :2])
+        else:
+            d = np.linalg.det(L)
+            ok = flow & (np.abs(d) > 1e-8)
+            M[ok] = np.linalg.inv(L[ok])
+        self.Lc = M
+
+    def corr(self, i, gw):
+        """kernel_deriv_corr (solver_sph_base.py:375-384)."""
+        if self.d.get("kernel_corr", 0) != 1:
+            return gw
+        return np.einsum("nab,nb->na", self.Lc[i], gw)
 
     def init_tmp(self):
         real = self.t > 0
@@ -151,7 +178,7 @@ class NumpySPH:
         gw = self.gradW(xij, r)
         sel = (self.t[i] == FLUID)
         u = self.vt[i] - self.vt[j]
-        dd = self._sum(i, np.where(sel, self.mV[j] * np.einsum("ij,ij->i", u, gw), 0.0))
+        dd = self._sum(i, np.where(sel, self.mV[j] * np.einsum("ij,ij->i", u, self.corr(i, gw)), 0.0))
         tj = self.t[j]
         vis_ok = (tj == FLUID) | is_bdy(tj) | (tj == RIGID)
         c1 = 2 * (self.dim + 2) * self.d["viscosity"]
@@ -178,7 +205,7 @@ class NumpySPH:
         return self._sum(i, fac[:, None] * xij)
 
     def _vgrad(self, i, j, xij, r, sel):
-        gw = self.gradW(xij, r)
+        gw = self.corr(i, self.gradW(xij, r))
         du = (self.vt[j] - self.vt[i]) * self.mV[j][:, None]
         L = self._sum(i, np.where(sel[:, None, None], du[:, :, None] * gw[:, None, :], 0.0))
         dd = self._sum(i, np.where(sel, self.mV[j] * np.einsum("ij,ij->i", self.vt[i] - self.vt[j], gw), 0.0))

@@ -37,7 +37,7 @@ enum {
   F_POS = 0, F_VEL, F_DENSITY, F_MASS, F_M_V, F_PRESSURE, F_TYPE, F_ID0,
   F_OBJ_ID, F_STRESS, F_STRESS_TMP, F_D_STRESS, F_STRAIN_EQU, F_STRAIN_EQU_P,
   F_D_STRAIN_EQU, F_D_STRAIN_EQU_P, F_FLAG_RETMAP, F_CSPM_F, F_D_DENSITY,
-  F_D_VEL, F_DENSITY_TMP, F_V_TMP, F_CELL, F_OFFSET, F_CELL_START
+  F_D_VEL, F_DENSITY_TMP, F_V_TMP, F_CELL, F_OFFSET, F_CELL_START, F_CSPM_L
 };
 enum {
   P_SORT = 0, P_SHEPARD, P_INIT_TMP, P_ONE_STEP, P_LF_HALF, P_LF_FULL,
@@ -97,6 +97,7 @@ struct Sim {
   std::vector<R> drhoRK, dvRKx, dvRKy, dvRKz;
   std::vector<double> dsigRK[6];
   std::vector<R> xsx, xsy, xsz;
+  std::vector<R> Lc[9];  // CSPM_L (row-major 3x3), kernelCorrection 1
   std::vector<int64_t> start;  // size C+1, exclusive scan of cell counts
 
   bool f32_state() const { return sizeof(R) == 4; }
@@ -149,6 +150,7 @@ struct Sim {
       zr(*v);
     for (int c = 0; c < 6; ++c) { zd(sig[c]); zd(sigt[c]); zd(dsig[c]); zd(dsigRK[c]); }
     zd(seq); zd(seqp); zd(dseq); zd(dseqp);
+    for (int c = 0; c < 9; ++c) Lc[c].assign(n, (c % 4 == 0) ? R(1) : R(0));
     start.assign(C + 1, 0);
   }
 
@@ -288,9 +290,72 @@ struct Sim {
     for (auto* v : {&seq, &seqp, &dseq, &dseqp}) apply(*v, perm);
   }
 
+  // ------------------------------------------------ kernel correction
+  // calc_CSPM_L (solver_sph_base.py:400-423): for flow particles
+  // L = sum_{j same type} m_V_j (x_j - x_i) (x) grad W_ij, stored inverted
+  // when |det| > eps (2D: the upper 2x2 block, the rest zero), else I;
+  // other particles keep I.
+  void calc_cspm_L() {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+      R M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      if (is_flow(type[i])) {
+        R L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        const int ti = type[i];
+        walk(i, [&](int64_t j, R x, R y, R z, R r2) {
+          if (type[j] != ti) return;
+          R f = gradW_fac(std::sqrt(r2));
+          R g[3] = {f * x, f * y, f * z};
+          R c[3] = {mV[j] * (-x), mV[j] * (-y), mV[j] * (-z)};
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) L[a][b] += c[a] * g[b];
+        });
+        invert_corr(L, M);
+      }
+      for (int c = 0; c < 9; ++c) Lc[c][i] = M[c];
+    }
+  }
+  // The inverse of calc_CSPM_L: adjugate times 1/det (row-major out).
+  void invert_corr(const R L[3][3], R M[9]) const {
+    if (dim == 2) {
+      R det = L[0][0] * L[1][1] - L[0][1] * L[1][0];
+      if (std::abs(det) > eps) {
+        R id = R(1) / det;
+        R o[9] = {L[1][1] * id, -L[0][1] * id, 0, -L[1][0] * id, L[0][0] * id, 0, 0, 0, 0};
+        for (int c = 0; c < 9; ++c) M[c] = o[c];
+      }
+      return;
+    }
+    R c00 = L[1][1] * L[2][2] - L[1][2] * L[2][1];
+    R c01 = L[1][2] * L[2][0] - L[1][0] * L[2][2];
+    R c02 = L[1][0] * L[2][1] - L[1][1] * L[2][0];
+    R det = L[0][0] * c00 + L[0][1] * c01 + L[0][2] * c02;
+    if (std::abs(det) > eps) {
+      R id = R(1) / det;
+      R o[9] = {c00 * id, (L[0][2] * L[2][1] - L[0][1] * L[2][2]) * id, (L[0][1] * L[1][2] - L[0][2] * L[1][1]) * id,
+                c01 * id, (L[0][0] * L[2][2] - L[0][2] * L[2][0]) * id, (L[0][2] * L[1][0] - L[0][0] * L[1][2]) * id,
+                c02 * id, (L[0][1] * L[2][0] - L[0][0] * L[2][1]) * id, (L[0][0] * L[1][1] - L[0][1] * L[1][0]) * id};
+      for (int c = 0; c < 9; ++c) M[c] = o[c];
+    }
+  }
+  // kernel_deriv_corr (solver_sph_base.py:375-384): CSPM_L_i grad W with
+  // kernelCorrection 1, grad W unchanged otherwise.
+  void corr(int64_t i, R g[3]) const {
+    if (P.kernel_corr != 1) return;
+    R o[3];
+    for (int a = 0; a < 3; ++a) o[a] = (Lc[3 * a][i] * g[0] + Lc[3 * a + 1][i] * g[1]) + Lc[3 * a + 2][i] * g[2];
+    for (int a = 0; a < 3; ++a) g[a] = o[a];
+  }
+
   // ------------------------------------------------------------ Shepard
-  // calc_CSPM_f (solver_sph_base.py:386-398): 1 / sum_flow m_V_j W_ij.
+  // calc_kernel_corr (solver_sph_base.py:363-368): calc_CSPM_f, then
+  // calc_CSPM_L with kernelCorrection 1.
   void shepard() {
+    calc_cspm_f();
+    if (P.kernel_corr == 1) calc_cspm_L();
+  }
+  // calc_CSPM_f (solver_sph_base.py:386-398): 1 / sum_flow m_V_j W_ij.
+  void calc_cspm_f() {
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; ++i) {
       R s = R(0);
@@ -374,7 +439,9 @@ struct Sim {
       R f = gradW_fac(rn);
       R gx = f * x, gy = f * y, gz = f * z;
       R ux = vtx[i] - vtx[j], uy = vty[i] - vty[j], uz = vtz[i] - vtz[j];
-      dd += (mV[j] * ux) * gx + (mV[j] * uy) * gy + (mV[j] * uz) * gz;
+      R cg[3] = {gx, gy, gz};  // d_density takes the corrected gradient
+      corr(i, cg);
+      dd += (mV[j] * ux) * cg[0] + (mV[j] * uy) * cg[1] + (mV[j] * uz) * cg[2];
       R vis = 0;
       int tj = type[j];
       if (tj == FLUID || is_bdy(tj) || tj == RIGID) {
@@ -416,6 +483,7 @@ struct Sim {
     walk(i, [&](int64_t j, R x, R y, R z, R r2) {
       R f = gradW_fac(std::sqrt(r2));
       R gw[3] = {f * x, f * y, f * z};
+      corr(i, gw);
       R u[3] = {mV[j] * (vtx[j] - vtx[i]), mV[j] * (vty[j] - vty[i]), mV[j] * (vtz[j] - vtz[i])};
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) L[a][b] += u[a] * gw[b];
@@ -488,6 +556,7 @@ struct Sim {
     walk(i, [&](int64_t j, R x, R y, R z, R r2) {
       R f = gradW_fac(std::sqrt(r2));
       R gw[3] = {f * x, f * y, f * z};
+      corr(i, gw);
       R rj2 = rhot[j] * rhot[j];
       R s = mV[j] * rhot[j];
       for (int a = 0; a < 3; ++a) {
@@ -655,6 +724,7 @@ struct Sim {
     walk(i, [&](int64_t j, R x, R y, R z, R r2) {
       R f = gradW_fac(std::sqrt(r2));
       R gw[3] = {f * x, f * y, f * z};
+      corr(i, gw);
       R u[3] = {mV[j] * (vtx[j] - vtx[i]), mV[j] * (vty[j] - vty[i]), mV[j] * (vtz[j] - vtz[i])};
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) L[a][b] += u[a] * gw[b];
@@ -1006,6 +1076,10 @@ struct Sim {
       case F_CELL: for (int64_t i = 0; i < n; ++i) iv[i] = (int32_t)flat(cx[i], cy[i], cz[i]); return 0;
       case F_OFFSET: g3(ox, oy, oz); return 0;
       case F_CELL_START: for (int64_t c = 0; c <= C; ++c) iv[c] = (int32_t)start[c]; return 0;
+      case F_CSPM_L:
+        for (int64_t i = 0; i < n; ++i)
+          for (int c = 0; c < 9; ++c) d[9 * i + c] = (double)Lc[c][i];
+        return 0;
     }
     return -1;
   }

@@ -18,14 +18,14 @@ def scene_cfg(name):
 
 
 def tiny_cfg(method=1, dim=2, xsph=False, r=None, domain=(0.2, 0.2, 0.06), block=(0.08, 0.08, 0.04),
-             boundary=2, ti=2, kernel=1):
+             boundary=2, ti=2, kernel=1, kcorr=0):
     """A small dam-break / column scene for brute-force checks."""
     base = json.load(open(os.path.join(SCENES, "dam_break_2d.json" if method == 1 else
                                        "column_collapse_mui.json")))
     c = base["Configuration"]
     c.update({"domainEnd": list(domain), "particleRadius": r or (0.005 if dim == 2 else 0.01),
               "is2D": dim == 2, "simulationMethod": method, "xsph": xsph, "boundary": boundary,
-              "timeIntegration": ti, "kernel": kernel})
+              "timeIntegration": ti, "kernel": kernel, "kernelCorrection": kcorr})
     base["Blocks"][0]["size"] = list(block)
     return SimConfiger(config=base)
 

@@ -37,6 +37,9 @@ CASES = {
     "wc2d_rep": ("dam_break_2d", {"boundary": 3}, 100, 0.6),
     "wc2d_dummy_rep": ("dam_break_2d", {"boundary": 4}, 100, 0.6),
     "mui2d_dummy_rep": ("column_collapse_mui", {"boundary": 4}, 30, 0.2),
+    "wc2d_cspm": ("dam_break_2d", {"kernelCorrection": 1}, 100, 0.6),
+    "mui2d_cspm": ("column_collapse_mui", {"kernelCorrection": 1}, 30, 0.2),
+    "dp2d_cspm": ("column_collapse_dp", {"kernelCorrection": 1}, 30, 0.2),
 }
 
 STATE = ["cell", "offset", "vel", "density", "mass", "m_V", "type", "id0", "obj_id", "pressure",
@@ -152,6 +155,7 @@ def test_sort_bit_exact_and_neighbor_sets(case):
 
 def test_single_pass_parity(case):
     key, sc, orc, H = case
+    extra = {"shepard": ["cspm_L"]} if sc.kernel_corr == 1 else {}
     eng, slot = new_engine(sc)
     copy_state(orc, eng, slot, sc.method)
     o2 = Oracle(sc.desc(), 32)
@@ -161,7 +165,7 @@ def test_single_pass_parity(case):
                "advect_pos", "advect_something"]:
         o2.run(ps)
         eng.run(ps)
-        check_fields(sc, o2, eng, slot, H, pass_fields(sc.method, ps), ps)
+        check_fields(sc, o2, eng, slot, H, pass_fields(sc.method, ps) + extra.get(ps, []), ps)
     # everything after the full sequence
     check_fields(sc, o2, eng, slot, H, [f for f in state_fields(sc.method)
                                         if f not in ("cell", "type", "id0", "obj_id", "flag_retmap")],

@@ -441,12 +441,14 @@ def test_dp_return_mapping_postconditions():
 
 
 # ------------------------------------------------ numpy restatement vs C++
-@pytest.mark.parametrize("method,dim,xsph,boundary", [
-    (1, 2, False, 2), (1, 3, False, 2), (2, 2, True, 2), (3, 2, True, 2), (3, 3, False, 2),
-    (2, 3, False, 2), (1, 2, True, 2), (1, 2, False, 3), (1, 2, False, 4), (2, 2, False, 4),
-    (3, 2, False, 4), (1, 3, False, 4)])
-def test_oracle_matches_numpy_restatement(method, dim, xsph, boundary):
-    sc = build_scene(tiny_cfg(method=method, dim=dim, xsph=xsph, boundary=boundary))
+@pytest.mark.parametrize("method,dim,xsph,boundary,kcorr", [
+    (1, 2, False, 2, 0), (1, 3, False, 2, 0), (2, 2, True, 2, 0), (3, 2, True, 2, 0),
+    (3, 3, False, 2, 0), (2, 3, False, 2, 0), (1, 2, True, 2, 0), (1, 2, False, 3, 0),
+    (1, 2, False, 4, 0), (2, 2, False, 4, 0), (3, 2, False, 4, 0), (1, 3, False, 4, 0),
+    (1, 2, False, 2, 1), (2, 2, False, 2, 1), (3, 2, False, 2, 1), (1, 3, False, 2, 1),
+    (2, 3, False, 2, 1)])
+def test_oracle_matches_numpy_restatement(method, dim, xsph, boundary, kcorr):
+    sc = build_scene(tiny_cfg(method=method, dim=dim, xsph=xsph, boundary=boundary, kcorr=kcorr))
     d = sc.desc()
     o = O.Oracle(d, 64)
     O.load_scene(o, sc)
@@ -471,6 +473,8 @@ def test_oracle_matches_numpy_restatement(method, dim, xsph, boundary):
         close(o.get("strain_equ"), ref.seq[idx])
     if method == 3:
         np.testing.assert_array_equal(o.get("flag_retmap"), ref.flag[idx])
+    if kcorr == 1:
+        close(o.get("cspm_L").reshape(-1, 3, 3), ref.Lc[idx], tol=1e-8)
 
 
 # ------------------------------------------------ repulsive particles
@@ -515,3 +519,29 @@ def test_repulsive_layer_count_and_placement():
     on_wall = (np.isclose(rep[:, 0], 0.0) | np.isclose(rep[:, 0], 3.22) | np.isclose(rep[:, 1], 0.0))
     assert np.all(on_wall)
     assert np.allclose(np.diff(np.sort(rep[np.isclose(rep[:, 0], 0.0), 1])), 0.005)
+
+
+# ------------------------------------------------ kernel correction (CSPM)
+@pytest.mark.parametrize("dim,kernel,scale", [(2, 1, 1.0), (2, 0, 1.0), (3, 0, 1.0), (3, 1, 1 / 8)])
+def test_cspm_L_interior_lattice(dim, kernel, scale):
+    """calc_CSPM_L (solver_sph_base.py:400-423): for a particle deep inside a
+    uniform lattice, sum m_V (x_j - x_i) (x) grad W is the discrete
+    gradient of x, i.e. (integral of W) * I, so its stored inverse is I
+    (I / 8 in 3D Wendland C2, whose kept constant integrates to 8); the
+    off-diagonal terms vanish by symmetry.  Boundary particles keep I."""
+    cfg = tiny_cfg(method=1, dim=dim, kernel=kernel, kcorr=1, block=(0.1, 0.1, 0.1),
+                   domain=(0.2, 0.2, 0.2), r=0.005 if dim == 2 else 0.01)
+    sc = build_scene(cfg)
+    o = O.Oracle(sc.desc(), 64)
+    O.load_scene(o, sc)
+    o.run("sort")
+    o.run("shepard")
+    M = o.get("cspm_L").reshape(-1, 3, 3)
+    x = o.get("pos")
+    t = o.get("type")
+    centre = np.array([0.05, 0.05, 0.05 if dim == 3 else 0.0])
+    i = int(np.argmin(np.where(t == 1, np.linalg.norm(x - centre, axis=1), 1e9)))
+    eye = np.eye(3) if dim == 3 else np.diag([1.0, 1.0, 0.0])
+    np.testing.assert_allclose(M[i], scale * eye, atol=0.02 * scale)
+    assert np.max(np.abs(M[i] - np.diag(np.diag(M[i])))) < 1e-9
+    np.testing.assert_array_equal(M[t == -1], np.tile(np.eye(3), (int((t == -1).sum()), 1, 1)))

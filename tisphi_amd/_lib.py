@@ -23,7 +23,8 @@ FIELDS = dict(pos=(0, "f8", 3), vel=(1, "f8", 3), density=(2, "f8", 1), mass=(3,
               d_strain_equ=(14, "f8", 1), d_strain_equ_p=(15, "f8", 1),
               flag_retmap=(16, "i4", 1), cspm_f=(17, "f8", 1), d_density=(18, "f8", 1),
               d_vel=(19, "f8", 3), density_tmp=(20, "f8", 1), v_tmp=(21, "f8", 3),
-              cell=(22, "i4", 1), offset=(23, "f8", 3), cell_start=(24, "i4", 1))
+              cell=(22, "i4", 1), offset=(23, "f8", 3), cell_start=(24, "i4", 1),
+              cspm_L=(25, "f8", 9))
 PASSES = dict(sort=0, shepard=1, init_tmp=2, one_step=3, lf_half=4, lf_full=5,
               advect_pos=6, advect_something=7, step=8, se=9, enforce_boundary=10)
 

@@ -42,6 +42,7 @@ struct SceneDev {
   float g, rs2, h1, kW, kC, eps;
   float kG, kGc;  // gradient factors: -5 kW h1^2 (Wendland C2), kC h1^2 (cubic)
   float rs2c;     // stencil-cell culling threshold 1.0001 rs2
+  int kcorr;      // kernelCorrection: 1 = CSPM (corrected kernel gradients)
   int rep;        // add the repulsive-particle force (boundary 3 / 4; WC, muI)
   float rep_k, rep_d, rep_h;  // 0.01 c^2, particle diameter, 0.75 h
   float dt, half_dt, dt6, rho0, visc_c1, visc_c2, grav[3];
@@ -74,6 +75,7 @@ struct PWork {
   int32_t* tmp;      // scan position -> particle
   int32_t* sid;      // particle -> scene slot (static per scene range)
   float* p;          // pressure
+  float* Lc;          // inverse CSPM_L, row-major 3x3, entry c at [c*cap + p]
   float4* vp;        // WC pair record of a particle: v_tmp, p / density_tmp^2;
                      // sign bit of .w set for a boundary particle
   float* cspm;       // Shepard factor
@@ -177,6 +179,60 @@ __device__ __forceinline__ float kernel_gradfac_t(const SceneDev& S, float rn) {
     f = q <= 1.0f ? fi : fo;
   }
   return (rn > S.eps && q <= 2.0f) ? f : 0.0f;
+}
+
+// kernel_deriv_corr (solver_sph_base.py:375-384) with kernelCorrection 1:
+// the gradient times the particle's stored inverse CSPM_L.
+__device__ __forceinline__ void corr_load(const float* __restrict__ Lc, int stride, int p,
+                                          float m[9]) {
+#pragma unroll
+  for (int c = 0; c < 9; ++c) m[c] = Lc[(size_t)c * stride + p];
+}
+__device__ __forceinline__ void corr_apply(const float m[9], float g[3]) {
+  const float o0 = (m[0] * g[0] + m[1] * g[1]) + m[2] * g[2];
+  const float o1 = (m[3] * g[0] + m[4] * g[1]) + m[5] * g[2];
+  const float o2 = (m[6] * g[0] + m[7] * g[1]) + m[8] * g[2];
+  g[0] = o0;
+  g[1] = o1;
+  g[2] = o2;
+}
+// Inverse of calc_CSPM_L (solver_sph_base.py:410-423): adjugate times 1/det
+// when |det| > eps (2D: the upper 2x2 block, the rest zero); M keeps I
+// otherwise.
+template <int DIM>
+__device__ __forceinline__ void corr_invert(const float L[3][3], float eps, float M[9]) {
+  if (DIM == 2) {
+    const float det = L[0][0] * L[1][1] - L[0][1] * L[1][0];
+    if (fabsf(det) > eps) {
+      const float id = 1.0f / det;
+      M[0] = L[1][1] * id;
+      M[1] = -L[0][1] * id;
+      M[2] = 0.0f;
+      M[3] = -L[1][0] * id;
+      M[4] = L[0][0] * id;
+      M[5] = 0.0f;
+      M[6] = 0.0f;
+      M[7] = 0.0f;
+      M[8] = 0.0f;
+    }
+    return;
+  }
+  const float c00 = L[1][1] * L[2][2] - L[1][2] * L[2][1];
+  const float c01 = L[1][2] * L[2][0] - L[1][0] * L[2][2];
+  const float c02 = L[1][0] * L[2][1] - L[1][1] * L[2][0];
+  const float det = L[0][0] * c00 + L[0][1] * c01 + L[0][2] * c02;
+  if (fabsf(det) > eps) {
+    const float id = 1.0f / det;
+    M[0] = c00 * id;
+    M[1] = (L[0][2] * L[2][1] - L[0][1] * L[2][2]) * id;
+    M[2] = (L[0][1] * L[1][2] - L[0][2] * L[1][1]) * id;
+    M[3] = c01 * id;
+    M[4] = (L[0][0] * L[2][2] - L[0][2] * L[2][0]) * id;
+    M[5] = (L[0][2] * L[1][0] - L[0][0] * L[1][2]) * id;
+    M[6] = c02 * id;
+    M[7] = (L[0][1] * L[2][0] - L[0][0] * L[2][1]) * id;
+    M[8] = (L[0][0] * L[1][1] - L[0][1] * L[1][0]) * id;
+  }
 }
 
 // Repulsive-particle force calc_repulsive_force (solver_sph_base.py:676-690):

@@ -82,6 +82,16 @@ struct Timing {
 
 }  // namespace
 
+// Options of the step that the host decides from the resident scenes (a
+// captured graph is valid only for the options it was captured with).
+struct StepFlags {
+  int ti = 2;
+  bool xsph = false, collide = false, kcorr = false;
+  bool operator==(const StepFlags& o) const {
+    return ti == o.ti && xsph == o.xsph && collide == o.collide && kcorr == o.kcorr;
+  }
+};
+
 struct tsph_engine {
   int method = 1, dim = 2, max_scenes = 1, device = 0;
   int32_t cap = 0, ccap = 0, tiles_alloc = 0;
@@ -97,6 +107,7 @@ struct tsph_engine {
   int32_t nl_cap = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool graph_valid = false;
+  StepFlags graph_flags;  // options the graphs were captured with
   int parity = 0;  // which state buffer is current
   PState buf[2];
   PWork w;
@@ -166,7 +177,7 @@ int alloc_work(tsph_engine* E) {
   const size_t nb = (c + kScanTile - 1) / kScanTile;
   int r;
   if ((r = dalloc(&w.key, n, b)) || (r = dalloc(&w.slot, n, b)) || (r = dalloc(&w.tmp, n, b)) ||
-      (r = dalloc(&w.sid, n, b)) || (r = dalloc(&w.p, n, b)) || (r = dalloc(&w.vp, n, b)) ||
+      (r = dalloc(&w.sid, n, b)) || (r = dalloc(&w.p, n, b)) || (r = dalloc(&w.vp, n, b)) || (r = dalloc(&w.Lc, 9 * n, b)) ||
       (r = dalloc(&w.cspm, n, b)) ||
       (r = dalloc(&w.drho, n, b)) || (r = dalloc(&w.dv, n, b)) || (r = dalloc(&w.xs, n, b)) ||
       (r = dalloc(&w.drho_rk, n, b)) || (r = dalloc(&w.dv_rk, n, b)) ||
@@ -190,7 +201,7 @@ int alloc_work(tsph_engine* E) {
 }
 
 void free_work(PWork& w) {
-  void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.vp, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
+  void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.vp, w.Lc, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
                   w.dsig_d, w.dseq, w.dseqp, w.xs, w.drho_rk, w.dv_rk, w.dsig_rk, w.count,
                   w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc, w.cflow};
   for (void* p : ptrs)
@@ -254,6 +265,8 @@ void resolve_timing(tsph_engine* E) {
   T.pending.clear();
 }
 
+StepFlags step_flags(tsph_engine* E);
+
 template <int DIM>
 struct Seq {
   tsph_engine* E;
@@ -294,6 +307,11 @@ struct Seq {
       L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
     } else {
       L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
+    }
+    // calc_kernel_corr (solver_sph_base.py:363-368): CSPM_L after CSPM_f
+    if (step_flags(E).kcorr) {
+      if (E->nl_valid) L.go("cspm_L", k_cspm_L<DIM, true>, pb(), E->pair_block, E->args());
+      else L.go("cspm_L", k_cspm_L<DIM, false>, pb(), E->pair_block, E->args());
     }
   }
   void init_tmp() { L.go("init_tmp", k_init_tmp, tb(), E->elem_block, E->args()); }
@@ -420,10 +438,6 @@ struct Seq {
   }
 };
 
-struct StepFlags {
-  int ti = 2;
-  bool xsph = false, collide = false;
-};
 StepFlags step_flags(tsph_engine* E) {
   StepFlags f;
   bool first = true;
@@ -433,6 +447,7 @@ StepFlags step_flags(tsph_engine* E) {
     first = false;
     f.xsph = f.xsph || E->scenes[s].xsph;
     f.collide = f.collide || E->scenes[s].boundary == 1;
+    f.kcorr = f.kcorr || E->scenes[s].kcorr == 1;
   }
   return f;
 }
@@ -475,6 +490,7 @@ int build_graphs(tsph_engine* E) {
   }
   E->parity = p0;
   E->graph_valid = true;
+  E->graph_flags = step_flags(E);
   return TSPH_OK;
 }
 
@@ -506,7 +522,8 @@ int validate_desc(const tsph_engine* E, const tsph_scene_desc* d) {
   if (d->dim != E->dim) return fail(TSPH_ERR_INVALID_ARG, "scene dimension differs from the engine's");
   if (d->boundary < 0 || d->boundary > 4) return fail(TSPH_ERR_INVALID_ARG, "boundary must be 0..4");
   if (d->kernel != 0 && d->kernel != 1) return fail(TSPH_ERR_INVALID_ARG, "kernel must be 0 or 1");
-  if (d->kernel_corr != 0) return fail(TSPH_ERR_UNSUPPORTED, "kernelCorrection != 0 is not built");
+  if (d->kernel_corr != 0 && d->kernel_corr != 1)
+    return fail(TSPH_ERR_UNSUPPORTED, "kernelCorrection 2 (MLS) is not built");
   if (d->time_integration != 1 && d->time_integration != 2 && d->time_integration != 4)
     return fail(TSPH_ERR_UNSUPPORTED, "timeIntegration must be 1, 2 or 4");
   if (d->n_particles < 0) return fail(TSPH_ERR_INVALID_ARG, "negative particle count");
@@ -550,6 +567,7 @@ SceneDev make_scene(const tsph_engine* E, const tsph_scene_desc* d) {
   S.rs2c = S.rs2 * 1.0001f;
   // repulsive layer (boundary 3 / 4): WC and muI add its force; the DP
   // solver does not (solver_sph_dp.py:262-264 leaves it out)
+  S.kcorr = d->kernel_corr;
   S.rep = (d->boundary == 3 || d->boundary == 4) && d->method != kDP;
   S.rep_k = (float)(0.01 * d->vsound * d->vsound);
   S.rep_d = (float)d->particle_diameter;
@@ -602,6 +620,8 @@ FieldInfo field_info(int f) {
       return {3, false};
     case TSPH_F_STRESS: case TSPH_F_STRESS_TMP: case TSPH_F_D_STRESS:
       return {6, false};
+    case TSPH_F_CSPM_L:
+      return {9, false};
     case TSPH_F_TYPE: case TSPH_F_ID0: case TSPH_F_OBJ_ID: case TSPH_F_FLAG_RETMAP: case TSPH_F_CELL:
       return {1, true};
     default:
@@ -780,6 +800,15 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
     case TSPH_F_MASS: return f1(cur.mass);
     case TSPH_F_PRESSURE: return f1(w.p);
     case TSPH_F_CSPM_F: return f1(w.cspm);
+    case TSPH_F_CSPM_L: {
+      if (up) return fail(TSPH_ERR_INVALID_ARG, "cspm_L is download-only");
+      for (int c = 0; c < 9; ++c) {
+        std::vector<float> v;
+        if ((r = d2h(E, (const float*)(w.Lc + (size_t)c * st), off, n, v))) return r;
+        for (int64_t i = 0; i < n; ++i) hd[9 * i + c] = v[i];
+      }
+      return TSPH_OK;
+    }
     case TSPH_F_D_DENSITY: return f1(w.drho);
     case TSPH_F_TYPE: {
       std::vector<int8_t> v;
@@ -1015,7 +1044,9 @@ int tsph_step(void* engine, int nsteps) {
   if (!E || nsteps < 0) return fail(TSPH_ERR_INVALID_ARG, "bad argument");
   if (E->active_count() == 0 || nsteps == 0) return TSPH_OK;
   if (E->use_graph && !E->timing.on) {
-    if (!E->graph_valid) {
+    // recapture when the residents ask for other step options than the
+    // graphs hold (a scene with XSPH, collisions or kernel correction added)
+    if (!E->graph_valid || !(E->graph_flags == step_flags(E))) {
       int r = build_graphs(E);
       if (r) return r;
     }

@@ -614,6 +614,39 @@ __global__ __launch_bounds__(256) void k_shepard_list(KArgs a, int skip_fluid) {
   a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
 }
 
+// calc_CSPM_L (solver_sph_base.py:400-423) for scenes with
+// kernelCorrection 1: L = sum_{j same type} m_V_j (x_j - x_i) (x) grad W_ij
+// over a flow particle's neighbours, stored inverted; every other particle
+// stores I.
+template <int DIM, bool NL>
+__global__ __launch_bounds__(256) void k_cspm_L(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev S = a.scenes[s];
+  if (S.kcorr != 1) return;
+  float M[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+  const int8_t* __restrict__ type = a.cur.type;
+  const int ti = type[p];
+  if (is_flow(ti)) {
+    const Centre C = centre_of(a, S, p);
+    float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                          float mVj, int) {
+      if (type[j] != ti) return;
+      const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
+      const float g[3] = {f * x, f * y, f * z};
+      const float c[3] = {mVj * (-x), mVj * (-y), mVj * (-z)};
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) L[aa][bb] += c[aa] * g[bb];
+    });
+    corr_invert<DIM>(L, S.eps, M);
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) a.w.Lc[(size_t)c * a.stride + p] = M[c];
+}
+
 // =================================================================== Shepard
 // calc_CSPM_f (solver_sph_base.py:386-398)
 template <int DIM>
@@ -708,7 +741,7 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
 // solver_sph_base.py:386-398: flow neighbours, list order), which the fused
 // step then leaves out of the Shepard pass: in WC only boundary particles
 // read CSPM_f within the step, so fluid particles get it here, bitwise equal.
-template <int DIM, bool NL, bool SHEP, int KT>
+template <int DIM, bool NL, bool SHEP, int KT, bool KC>
 __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, int p) {
   const Centre C = centre_of(a, S, p);
   // A WC scene holds fluid and boundary particles only (checked at upload):
@@ -719,6 +752,8 @@ __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, 
   const float pri = vp[p].w;
   const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
   const float rr = rho0 / vi.w;  // rho0 / rho_i of the boundary viscosity
+  float M[9];
+  if (KC) corr_load(a.w.Lc, a.stride, p, M);
   float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f, shep = 0.f;
   neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
     const float rn = sqrtf(r2);
@@ -727,7 +762,13 @@ __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, 
     const float4 q = vp[j];
     const bool bj = __float_as_int(q.w) < 0;  // boundary neighbour
     const float ux = vi.x - q.x, uy = vi.y - q.y, uz = vi.z - q.z;
-    dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
+    if (KC) {  // d_density takes the corrected gradient, the forces do not
+      float cg[3] = {gx, gy, gz};
+      corr_apply(M, cg);
+      dd += (mVj * ux) * cg[0] + (mVj * uy) * cg[1] + (mVj * uz) * cg[2];
+    } else {
+      dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
+    }
     if (SHEP) shep += bj ? 0.0f : mVj * kernel_W_t<KT>(S, rn);
     // viscosity_force: fluid and boundary neighbours (solver_sph_wc.py:33-45)
     const float vx_ = ux * x + uy * y + uz * z;
@@ -769,8 +810,13 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
     return;
   }
   const SceneDev S = a.scenes[s];
-  if (S.kernel == 1) wc_fluid_one<DIM, NL, SHEP, 1>(a, S, p);
-  else wc_fluid_one<DIM, NL, SHEP, 0>(a, S, p);
+  if (S.kcorr == 1) {
+    if (S.kernel == 1) wc_fluid_one<DIM, NL, SHEP, 1, true>(a, S, p);
+    else wc_fluid_one<DIM, NL, SHEP, 0, true>(a, S, p);
+  } else {
+    if (S.kernel == 1) wc_fluid_one<DIM, NL, SHEP, 1, false>(a, S, p);
+    else wc_fluid_one<DIM, NL, SHEP, 0, false>(a, S, p);
+  }
 }
 
 // ======================================================================= muI
@@ -789,12 +835,16 @@ __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float4 vi = vtmp[p];
+  const bool kc = S.kcorr == 1;
+  float M[9];
+  if (kc) corr_load(a.w.Lc, a.stride, p, M);
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   float dd = 0.f;
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
     const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
-    const float gw[3] = {f * x, f * y, f * z};
+    float gw[3] = {f * x, f * y, f * z};
+    if (kc) corr_apply(M, gw);
     const float4 vj = vtmp[j];
     const float u[3] = {mVj * (vj.x - vi.x), mVj * (vj.y - vi.y), mVj * (vj.z - vi.z)};
 #pragma unroll
@@ -888,9 +938,7 @@ __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
 template <int DIM>
 __device__ __forceinline__ void soil_dv_pair(const SceneDev& S, const float* __restrict__ sgt,
                                              int st, int j, const float si[6], float mVj,
-                                             float rtj, float f, float x, float y, float z,
-                                             float dv[3]) {
-  const float gw[3] = {f * x, f * y, f * z};
+                                             float rtj, const float gw[3], float dv[3]) {
   const float rj2 = rtj * rtj;
   const float sc = mVj * rtj;
   float sj[6];
@@ -933,10 +981,15 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float dv[3] = {0.f, 0.f, 0.f};
+  const bool kc = S.kcorr == 1;
+  float M[9];
+  if (kc) corr_load(a.w.Lc, a.stride, p, M);
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
     const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
-    soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vtmp[j].w, f, x, y, z, dv);
+    float gw[3] = {f * x, f * y, f * z};
+    if (kc) corr_apply(M, gw);
+    soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vtmp[j].w, gw, dv);
   });
   if (S.rep) {
     // calc_rep_force_task (solver_sph_muI.py:34-36, 121-123), after the
@@ -1172,10 +1225,14 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   float dd = 0.f, dv[3] = {0.f, 0.f, 0.f};
+  const bool kc = S.kcorr == 1;
+  float M[9];
+  if (kc) corr_load(a.w.Lc, a.stride, p, M);
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
     const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
-    const float gw[3] = {f * x, f * y, f * z};
+    float gw[3] = {f * x, f * y, f * z};
+    if (kc) corr_apply(M, gw);
     const float4 vj = vtmp[j];
     const float u[3] = {mVj * (vj.x - vi.x), mVj * (vj.y - vi.y), mVj * (vj.z - vi.z)};
 #pragma unroll
@@ -1184,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
       for (int bb = 0; bb < 3; ++bb) L[aa][bb] += u[aa] * gw[bb];
     dd += (mVj * (vi.x - vj.x)) * gw[0] + (mVj * (vi.y - vj.y)) * gw[1] +
           (mVj * (vi.z - vj.z)) * gw[2];
-    soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vj.w, f, x, y, z, dv);
+    soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vj.w, gw, dv);
   });
   a.w.drho[p] = dd * vi.w;
   double Ld[3][3], sd[6], ds[6], dq, dqp;
