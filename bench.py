@@ -141,8 +141,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--no-granular", action="store_true", help="skip the granular (mu(I), DP) measurement")
-    ap.add_argument("--nl", type=int, default=1, choices=(0, 1, 2),
-                    help="neighbour search: 0 stencil walk per pass, 1 lists (default), 2 LDS-built lists")
+    ap.add_argument("--nl", type=int, default=2, choices=(0, 1, 2),
+                    help="neighbour search: 0 stencil walk per pass, 1 lists, 2 LDS-built lists (default)")
     ap.add_argument("--block", type=str, default="256,256", help="pair,elem threads per block")
     ap.add_argument("--tune", action="store_true",
                     help="pick block sizes and list mode with the autotuner (untimed steps) instead")

@@ -204,10 +204,10 @@ int tsph_set_graph(void* engine, int enable);
 /* Threads per block of the pair kernels and of the per-particle kernels
  * (64, 128 or 256). */
 int tsph_set_block(void* engine, int pair_block, int elem_block);
-/* Neighbour search mode: 1 (default) records per-step neighbour lists in
- * one stencil walk and replays them in the later pair passes of the step;
- * 2 does the same with the walk reading LDS-staged cell windows; 0 walks
- * the cell stencil in every pass.  All visit the same pairs in the same
+/* Neighbour search mode: 2 (default) records per-step neighbour lists in
+ * one stencil walk that reads LDS-staged cell windows and replays them in
+ * the later pair passes of the step; 1 does the same with the walk reading
+ * global memory; 0 walks the cell stencil in every pass.  All visit the same pairs in the same
  * order (for_all_neighbors, eng/particle_system.py:259-269). */
 int tsph_set_neighbor_list(void* engine, int enable);
 

@@ -34,7 +34,7 @@ def test_candidates():
     full = autotune.candidates(quick=False)
     assert len(full) == 18 and len(set(full)) == 18
     q = autotune.candidates(quick=True)
-    assert q[0] == (256, 256, 1) and len(q) == 6 and (256, 256, 2) in q
+    assert q[0] == (256, 256, 2) and len(q) == 6 and (256, 256, 1) in q
 
 
 def test_key_buckets():

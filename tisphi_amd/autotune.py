@@ -63,9 +63,9 @@ NL_MODES = (1, 2)  # lists built from global memory / from LDS windows
 
 def candidates(quick=False):
     """Shapes to try: (pair_block, elem_block, list mode).  ``quick`` varies
-    one at a time around (256, 256, 1)."""
+    one at a time around (256, 256, 2)."""
     if quick:
-        return [(256, 256, 1), (128, 256, 1), (64, 256, 1), (256, 128, 1), (256, 64, 1), (256, 256, 2)]
+        return [(256, 256, 2), (128, 256, 2), (64, 256, 2), (256, 128, 2), (256, 64, 2), (256, 256, 1)]
     return [(p, e, m) for m in NL_MODES for p in BLOCKS for e in BLOCKS]
 
 

@@ -100,7 +100,7 @@ struct tsph_engine {
   int pair_block = 256, elem_block = 256;
   bool use_graph = true;
   bool use_nl = true;    // build per-step neighbour lists
-  bool nl_lds = false;   // build them from LDS-staged cell windows
+  bool nl_lds = true;    // build them from LDS-staged cell windows
   bool nl_valid = false; // lists match the current order and positions
   bool cflow_valid = false;  // per-cell flow flags match the current order
   bool fresh_sort = false;   // offsets renormalised, no motion since the sort

@@ -419,14 +419,14 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
 // ------------------------------------------------ LDS-staged list build
 // The same list build with the candidates read from LDS.  One wave per
 // workgroup owns one tile.  Cell keys run (row, fast) with the fast axis z
-// in 3D and y in 2D, so the stencil of every cell in a row segment
-// [fmin, fmax] is 3^(DIM-1) key intervals [row', fmin-1 .. fmax+1]; each is
-// one contiguous particle range.  The wave copies those ranges' positions
-// into LDS once and every lane walks its stencil there, in the order of
-// walk() (same pairs, same order, same values).  A tile whose particles
-// span two rows, or whose window exceeds the LDS budget, walks global
-// memory instead.
-constexpr int kLdsCand = 1280;  // staged candidates per wave (20 KiB)
+// in 3D and y in 2D; a tile's particles usually share one row, so for each
+// row offset (dx, dy in 3D; dx in 2D) the neighbour cells of all its lanes
+// are consecutive keys and their particles one contiguous range.  The wave
+// copies that range (coalesced, at most kStage particles) into LDS and every
+// lane walks its own three fast-axis cells there, in the order of walk()
+// (same pairs, same order, same values).  A tile spanning two rows, or a
+// larger range, reads global memory instead.
+constexpr int kStage = 256;  // staged candidates per row offset (4 KiB)
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -445,57 +445,9 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-template <int DIM, bool CULL, class Fn>
-__device__ __forceinline__ void walk_lds(const SceneDev& S, const int32_t* __restrict__ start,
-                                         const float4* spos, const int* rbase, int32_t shift, int i,
-                                         int cx, int cy, int cz, float oix, float oiy, float oiz,
-                                         Fn&& fn) {
-  const int gy = S.gn[1], gz = S.gn[2];
-#pragma unroll 1
-  for (int dx = -1; dx <= 1; ++dx) {
-    const int nx = cx + dx;
-    if (nx < 0 || nx >= S.gn[0]) continue;
-    const float bx = oix + (float)(cx - nx) * S.g;
-    const float ax = dx < 0 ? oix : (dx > 0 ? S.g - oix : 0.0f);
-#pragma unroll 1
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int ny = cy + dy;
-      if (ny < 0 || ny >= gy) continue;
-      const float by = oiy + (float)(cy - ny) * S.g;
-      const float ay = dy < 0 ? oiy : (dy > 0 ? S.g - oiy : 0.0f);
-      const int32_t rowkey = S.cbase + (nx * gy + ny) * gz;
-      const int lb = rbase[DIM == 3 ? (dx + 1) * 3 + (dy + 1) : (dx + 1)];
-#pragma unroll 1
-      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
-        const int nz = cz + dz;
-        if (DIM == 3 && (nz < 0 || nz >= gz)) continue;
-        if (CULL) {  // as in walk()
-          const float az = DIM == 3 ? (dz < 0 ? oiz : (dz > 0 ? S.g - oiz : 0.0f)) : 0.0f;
-          if (fmaf(az, az, fmaf(ay, ay, ax * ax)) > S.rs2c) continue;
-        }
-        const float bz = DIM == 3 ? oiz + (float)(cz - nz) * S.g : 0.0f;
-        const int32_t key = rowkey + nz;
-        const int code = ((dx + 1) << 4) | ((dy + 1) << 2) | (dz + 1);
-        const int32_t jb = start[key] + shift;
-        const int32_t je = start[key + 1] + shift;
-#pragma unroll 4
-        for (int32_t j = jb; j < je; ++j) {
-          const float4 pj = spos[j + lb];
-          const float x = bx - pj.x;
-          const float y = by - pj.y;
-          const float z = DIM == 3 ? bz - pj.z : 0.0f;
-          const float r2 = fmaf(z, z, fmaf(y, y, x * x));
-          if (r2 < S.rs2 && j != i) fn(j, x, y, z, r2, pj.w, code);
-        }
-      }
-    }
-  }
-}
-
 template <int DIM>
 __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int cull) {
-  __shared__ float4 spos[kLdsCand];
-  __shared__ int rbase[9];
+  __shared__ float4 sp[kStage];
   const int w = __builtin_amdgcn_readfirstlane((int)xcd_block());
   const int2 t = a.tiles[w];
   if (t.x < 0) return;  // unused tile: the whole workgroup leaves
@@ -503,71 +455,36 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
   const int lane = (int)threadIdx.x;
   const int p = t.y + lane;
   const SceneDev S = a.scenes[s];
-  const bool live = p < S.pbase + S.n;
   const int8_t* __restrict__ type = a.cur.type;
+  const float4* __restrict__ pos = a.cur.pos;
+  const int32_t* __restrict__ start = a.w.start;
   const int cap = a.nl_cap;
   const int32_t shift = scene_shift(a, S);
-  int cx = 0, cy = 0, cz = 0;
+  const int gx = S.gn[0], gy = S.gn[1], gz = S.gn[2];
+  bool act = p < S.pbase + S.n;
+  int cx = 0, cy = 0, cz = 0, key = 0;
   float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (live) {
-    decode_cell(S, a.cur.cell[p], cx, cy, cz);
-    P = a.cur.pos[p];
+  bool bdy = false;
+  if (act) {
+    const int32_t c = a.cur.cell[p];
+    decode_cell(S, c, cx, cy, cz);
+    key = S.cbase + c;
+    P = pos[p];
+    bdy = is_bdy(type[p]);
   }
-  const int gf = DIM == 3 ? S.gn[2] : S.gn[1];
-  const int row = DIM == 3 ? cx * S.gn[1] + cy : cx;
-  const int f = DIM == 3 ? cz : cy;
-  const int rmin = wave_min(live ? row : INT32_MAX), rmax = wave_max(live ? row : INT32_MIN);
-  const int fmin = wave_min(live ? f : INT32_MAX), fmax = wave_max(live ? f : INT32_MIN);
-  // Stencil rows of the segment and their staged particle ranges.
-  int total = 0;
-  bool staged = rmin == rmax;
-  const int fa = fmin - 1 > 0 ? fmin - 1 : 0;
-  const int fb = fmax + 1 < gf - 1 ? fmax + 1 : gf - 1;
-  int jb_k[9], cnt_k[9];
-  const int nrow = DIM == 3 ? 9 : 3;
-  if (staged) {
-    const int rx = DIM == 3 ? rmin / S.gn[1] : rmin;
-    const int ry = DIM == 3 ? rmin - rx * S.gn[1] : 0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      jb_k[k] = 0;
-      cnt_k[k] = 0;
-      if (k >= nrow) continue;
-      const int nx = rx + (DIM == 3 ? k / 3 : k) - 1;
-      const int ny = ry + (DIM == 3 ? k % 3 - 1 : 0);
-      if (nx < 0 || nx >= S.gn[0] || ny < 0 || ny >= S.gn[1]) continue;
-      const int32_t rk = S.cbase + (DIM == 3 ? (nx * S.gn[1] + ny) * S.gn[2] : nx * S.gn[1]);
-      jb_k[k] = a.w.start[rk + fa] + shift;
-      cnt_k[k] = a.w.start[rk + fb + 1] + shift - jb_k[k];
-      total += cnt_k[k];
-    }
-    staged = total <= kLdsCand;
-  }
-  if (staged) {
-    int off = 0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      if (k >= nrow) break;
-      for (int q = lane; q < cnt_k[k]; q += 64) spos[off + q] = a.cur.pos[jb_k[k] + q];
-      if (lane == 0) rbase[k] = off - jb_k[k];
-      off += cnt_k[k];
-    }
-    __syncthreads();
-  }
-  if (!live) return;
-  const bool bdy = is_bdy(type[p]);
-  if (bdy && use_cflow) {
+  if (act && bdy && use_cflow) {
+    // as in k_nl_build: no flow cell in reach, no neighbours to record
     bool any = false;
     for (int dx = -1; dx <= 1 && !any; ++dx) {
       const int nx = cx + dx;
-      if (nx < 0 || nx >= S.gn[0]) continue;
+      if (nx < 0 || nx >= gx) continue;
       for (int dy = -1; dy <= 1 && !any; ++dy) {
         const int ny = cy + dy;
-        if (ny < 0 || ny >= S.gn[1]) continue;
+        if (ny < 0 || ny >= gy) continue;
         for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
           const int nz = cz + dz;
-          if (nz < 0 || nz >= S.gn[2]) continue;
-          if (a.w.cflow[S.cbase + (nx * S.gn[1] + ny) * S.gn[2] + nz]) {
+          if (nz < 0 || nz >= gz) continue;
+          if (a.w.cflow[S.cbase + (nx * gy + ny) * gz + nz]) {
             any = true;
             break;
           }
@@ -576,25 +493,88 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
     }
     if (!any) {
       a.w.nlc[p] = 0;
-      return;
+      act = false;
     }
   }
-  uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
+  uint32_t* __restrict__ li = a.w.nl + nl_base(act ? p : t.y, cap);
   ListAppender ap{li, cap};
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
     if (bdy && !is_flow(type[j])) return;
     ap.add(nl_entry(j, code));
   };
-  if (staged && cull) {
-    walk_lds<DIM, true>(S, a.w.start, spos, rbase, shift, p, cx, cy, cz, P.x, P.y, P.z, rec);
-  } else if (staged) {
-    walk_lds<DIM, false>(S, a.w.start, spos, rbase, shift, p, cx, cy, cz, P.x, P.y, P.z, rec);
-  } else if (cull) {
-    walk<DIM, true>(S, a.w.start, a.cur.pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
-  } else {
-    walk<DIM>(S, a.w.start, a.cur.pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
+  const int row = DIM == 3 ? cx * gy + cy : cx;
+  const int rmin = wave_min(act ? row : INT32_MAX), rmax = wave_max(act ? row : INT32_MIN);
+  if (rmin > rmax) return;  // no lane left to walk (wave-uniform)
+  if (rmin != rmax) {
+    // two rows: each lane walks global memory
+    if (act) {
+      if (cull) walk<DIM, true>(S, start, pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
+      else walk<DIM>(S, start, pos, shift, p, cx, cy, cz, cx, cy, cz, P.x, P.y, P.z, rec);
+      a.w.nlc[p] = ap.finish();
+    }
+    return;
   }
-  a.w.nlc[p] = ap.finish();
+  // One row: rx (and ry in 3D) are the same for every active lane.  Walk
+  // order dx, dy, dz as in walk(); the row offsets (dx, dy in 3D; dx in 2D)
+  // are wave-uniform, and for each the three fast-axis neighbour cells of
+  // all lanes are one contiguous particle range, staged once.
+  const int rx = DIM == 3 ? rmin / gy : rmin;
+  const int ry = DIM == 3 ? rmin - rx * gy : 0;
+  const int f = DIM == 3 ? cz : cy;  // fast coordinate of the lane's cell
+  const int gf = DIM == 3 ? gz : gy;
+  const int flo = f > 0 ? -1 : 0, fhi = f < gf - 1 ? 1 : 0;
+#pragma unroll 1
+  for (int dx = -1; dx <= 1; ++dx) {
+    if (rx + dx < 0 || rx + dx >= gx) continue;
+    const float bx = P.x + (float)(-dx) * S.g;
+    const float ax = dx < 0 ? P.x : (dx > 0 ? S.g - P.x : 0.0f);
+#pragma unroll 1
+    for (int dy0 = (DIM == 3 ? -1 : 0); dy0 <= (DIM == 3 ? 1 : 0); ++dy0) {
+      if (DIM == 3 && (ry + dy0 < 0 || ry + dy0 >= gy)) continue;
+      // key of the lane's neighbour cell at fast offset 0 in this row offset
+      const int rkey = key + (DIM == 3 ? (dx * gy + dy0) * gz : dx * gy);
+      const int klo = wave_min(act ? rkey + flo : INT32_MAX);
+      const int khi = wave_max(act ? rkey + fhi : INT32_MIN);
+      const int32_t jlo = start[klo] + shift;
+      const int32_t cnt = start[khi + 1] + shift - jlo;
+      const bool staged = cnt <= kStage;  // wave-uniform
+      if (staged) {
+        __syncthreads();  // the previous row offset's reads are done
+        for (int q = lane; q < cnt; q += 64) sp[q] = pos[jlo + q];
+        __syncthreads();
+      }
+#pragma unroll 1
+      for (int df = -1; df <= 1; ++df) {
+        const int dy = DIM == 3 ? dy0 : df;
+        const int dz = DIM == 3 ? df : 0;
+        bool ok = act && f + df >= 0 && f + df < gf;
+        const float by = P.y + (float)(-dy) * S.g;
+        const float ay = dy < 0 ? P.y : (dy > 0 ? S.g - P.y : 0.0f);
+        if (ok && cull) {  // as in walk()
+          const float az = DIM == 3 ? (dz < 0 ? P.z : (dz > 0 ? S.g - P.z : 0.0f)) : 0.0f;
+          ok = !(fmaf(az, az, fmaf(ay, ay, ax * ax)) > S.rs2c);
+        }
+        if (!ok) continue;
+        const float bz = DIM == 3 ? P.z + (float)(-dz) * S.g : 0.0f;
+        const int nkey = rkey + df;
+        const int code = ((dx + 1) << 4) | ((dy + 1) << 2) | (dz + 1);
+        const int32_t jb = start[nkey] + shift, je = start[nkey + 1] + shift;
+        auto cand = [&](int32_t j, const float4& pj) {
+          const float x = bx - pj.x;
+          const float y = by - pj.y;
+          const float z = DIM == 3 ? bz - pj.z : 0.0f;
+          const float r2 = fmaf(z, z, fmaf(y, y, x * x));
+          if (r2 < S.rs2 && j != p) rec(j, x, y, z, r2, pj.w, code);
+        };
+        if (staged) {
+          for (int32_t j = jb; j < je; ++j) cand(j, sp[j - jlo]);
+        } else {
+          for (int32_t j = jb; j < je; ++j) cand(j, pos[j]);
+        }
+      }
+    }
+  }
+  if (act) a.w.nlc[p] = ap.finish();
 }
 
 // calc_CSPM_f (solver_sph_base.py:386-398) over the recorded lists.
