@@ -69,7 +69,7 @@ typedef struct tsph_scene_desc {
   int32_t method;           /* TSPH_METHOD_*                                 */
   int32_t boundary;         /* 0 none, 1 collision, 2 dummy (3/4 rejected)   */
   int32_t kernel;           /* 0 cubic spline, 1 Wendland C2                 */
-  int32_t kernel_corr;      /* 0 none, 1 CSPM (2 MLS rejected)               */
+  int32_t kernel_corr;      /* 0 none, 1 CSPM, 2 MLS switch (zero gradient)  */
   int32_t time_integration; /* 1 symplectic Euler, 2 leap-frog, 4 RK4        */
   int32_t xsph;             /* XSPH position update on/off                   */
   int32_t grid_num[3];      /* cells per axis; grid_num[2] == 1 in 2D        */
@@ -128,7 +128,8 @@ enum {
   TSPH_F_CELL_START = 24,   /* i32 x (cells+1), download only: scene-local
                                particle range of each cell                   */
   TSPH_F_CSPM_L = 25,       /* f64 x9, download only: inverse CSPM_L
-                               (row-major 3x3; kernelCorrection 1)          */
+                               (row-major 3x3; kernelCorrection 1; zero
+                               with 2, whose corrected gradient is 0)       */
   TSPH_F_COUNT = 26
 };
 

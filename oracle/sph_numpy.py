@@ -125,6 +125,8 @@ class NumpySPH:
         self.cspm = np.where(s != 0.0, 1.0 / np.where(s != 0, s, 1.0), 1.0)
         if self.d.get("kernel_corr", 0) == 1:
             self.cspm_L(i, j, xij, r)
+        elif self.d.get("kernel_corr", 0) == 2:
+            self.Lc = np.zeros((self.n, 3, 3))  # never computed: the zero field
 
     def cspm_L(self, i, j, xij, r):
         """calc_CSPM_L (solver_sph_base.py:400-423), inverted (np.linalg)."""
@@ -146,9 +148,13 @@ class NumpySPH:
         self.Lc = M
 
     def corr(self, i, gw):
-        """kernel_deriv_corr (solver_sph_base.py:375-384)."""
-        if self.d.get("kernel_corr", 0) != 1:
+        """kernel_deriv_corr (solver_sph_base.py:375-384): with
+        kernelCorrection 2 the branch is `pass`, leaving the zero vector."""
+        kc = self.d.get("kernel_corr", 0)
+        if kc == 0:
             return gw
+        if kc == 2:
+            return np.zeros_like(gw)
         return np.einsum("nab,nb->na", self.Lc[i], gw)
 
     def init_tmp(self):

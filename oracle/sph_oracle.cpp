@@ -339,9 +339,14 @@ struct Sim {
     }
   }
   // kernel_deriv_corr (solver_sph_base.py:375-384): CSPM_L_i grad W with
-  // kernelCorrection 1, grad W unchanged otherwise.
+  // kernelCorrection 1, the zero vector with 2 (its branch body is `pass`,
+  // so tmp keeps its initial 0), grad W unchanged with 0.
   void corr(int64_t i, R g[3]) const {
-    if (P.kernel_corr != 1) return;
+    if (P.kernel_corr == 0) return;
+    if (P.kernel_corr == 2) {
+      g[0] = g[1] = g[2] = R(0);
+      return;
+    }
     R o[3];
     for (int a = 0; a < 3; ++a) o[a] = (Lc[3 * a][i] * g[0] + Lc[3 * a + 1][i] * g[1]) + Lc[3 * a + 2][i] * g[2];
     for (int a = 0; a < 3; ++a) g[a] = o[a];
@@ -349,10 +354,13 @@ struct Sim {
 
   // ------------------------------------------------------------ Shepard
   // calc_kernel_corr (solver_sph_base.py:363-368): calc_CSPM_f, then
-  // calc_CSPM_L with kernelCorrection 1.
+  // calc_CSPM_L with kernelCorrection 1; with 2 nothing more (CSPM_L keeps
+  // the zero it was allocated with).
   void shepard() {
     calc_cspm_f();
     if (P.kernel_corr == 1) calc_cspm_L();
+    if (P.kernel_corr == 2)
+      for (int c = 0; c < 9; ++c) std::fill(Lc[c].begin(), Lc[c].end(), R(0));
   }
   // calc_CSPM_f (solver_sph_base.py:386-398): 1 / sum_flow m_V_j W_ij.
   void calc_cspm_f() {

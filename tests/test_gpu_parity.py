@@ -40,6 +40,8 @@ CASES = {
     "wc2d_cspm": ("dam_break_2d", {"kernelCorrection": 1}, 100, 0.6),
     "mui2d_cspm": ("column_collapse_mui", {"kernelCorrection": 1}, 30, 0.2),
     "dp2d_cspm": ("column_collapse_dp", {"kernelCorrection": 1}, 30, 0.2),
+    "wc2d_mls": ("dam_break_2d", {"kernelCorrection": 2}, 100, 0.6),
+    "mui2d_mls": ("column_collapse_mui", {"kernelCorrection": 2}, 30, 0.2),
 }
 
 STATE = ["cell", "offset", "vel", "density", "mass", "m_V", "type", "id0", "obj_id", "pressure",
@@ -155,7 +157,7 @@ def test_sort_bit_exact_and_neighbor_sets(case):
 
 def test_single_pass_parity(case):
     key, sc, orc, H = case
-    extra = {"shepard": ["cspm_L"]} if sc.kernel_corr == 1 else {}
+    extra = {"shepard": ["cspm_L"]} if sc.kernel_corr != 0 else {}
     eng, slot = new_engine(sc)
     copy_state(orc, eng, slot, sc.method)
     o2 = Oracle(sc.desc(), 32)

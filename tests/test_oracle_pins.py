@@ -446,7 +446,7 @@ def test_dp_return_mapping_postconditions():
     (3, 3, False, 2, 0), (2, 3, False, 2, 0), (1, 2, True, 2, 0), (1, 2, False, 3, 0),
     (1, 2, False, 4, 0), (2, 2, False, 4, 0), (3, 2, False, 4, 0), (1, 3, False, 4, 0),
     (1, 2, False, 2, 1), (2, 2, False, 2, 1), (3, 2, False, 2, 1), (1, 3, False, 2, 1),
-    (2, 3, False, 2, 1)])
+    (2, 3, False, 2, 1), (1, 2, False, 2, 2), (2, 2, False, 2, 2), (3, 2, False, 2, 2)])
 def test_oracle_matches_numpy_restatement(method, dim, xsph, boundary, kcorr):
     sc = build_scene(tiny_cfg(method=method, dim=dim, xsph=xsph, boundary=boundary, kcorr=kcorr))
     d = sc.desc()
@@ -475,6 +475,38 @@ def test_oracle_matches_numpy_restatement(method, dim, xsph, boundary, kcorr):
         np.testing.assert_array_equal(o.get("flag_retmap"), ref.flag[idx])
     if kcorr == 1:
         close(o.get("cspm_L").reshape(-1, 3, 3), ref.Lc[idx], tol=1e-8)
+    if kcorr == 2:
+        assert not np.any(o.get("cspm_L"))
+
+
+# ------------------------------------------------ kernelCorrection 2
+def test_mls_switch_zero_corrected_gradient():
+    """kernelCorrection 2 (solver_sph_base.py:363-384): calc_kernel_corr
+    computes nothing and kernel_deriv_corr's branch is empty, so every
+    corrected gradient is the zero vector.  WC: d_density (the only user,
+    :202) is 0 and the density never changes, while the forces (plain
+    gradient) still act.  mu(I): the velocity gradient (:188) and the stress
+    force (solver_sph_muI.py:42) vanish, so every soil particle falls freely,
+    v = g t exactly as the leap-frog integrates a constant acceleration."""
+    sc = build_scene(tiny_cfg(method=1, dim=2, kcorr=2))
+    o = O.Oracle(sc.desc(), 64)
+    O.load_scene(o, sc)
+    rho0 = o.get("density").copy()
+    o.step(5)
+    fl = o.get("type") == 1
+    assert not np.any(o.get("d_density")[fl])
+    np.testing.assert_array_equal(o.get("density")[fl], rho0[o.get("id0")][fl])
+    assert np.max(np.abs(o.get("d_vel")[fl, 0])) > 0  # pressure and viscosity still act
+    sc = build_scene(tiny_cfg(method=2, dim=2, kcorr=2))
+    o = O.Oracle(sc.desc(), 64)
+    O.load_scene(o, sc)
+    o.step(7)
+    soil = o.get("type") == 3
+    t = 7 * sc.desc()["dt"]
+    v = o.get("vel")[soil]
+    np.testing.assert_allclose(v[:, 1], -9.81 * t, rtol=1e-12)
+    assert not np.any(v[:, 0])
+    assert not np.any(o.get("d_density")[soil])
 
 
 # ------------------------------------------------ repulsive particles

@@ -137,10 +137,12 @@ def test_scene_json_sections_and_errors(tmp_path):
     cfg3["Configuration"]["boundary"] = 5
     with pytest.raises(NotImplementedError):
         build_scene(SimConfiger(config=cfg3))
-    cfg3["Configuration"]["kernelCorrection"] = 2
+    cfg3["Configuration"]["kernelCorrection"] = 3
     cfg3["Configuration"]["boundary"] = 2
     with pytest.raises(NotImplementedError):
         build_scene(SimConfiger(config=cfg3))
+    cfg3["Configuration"]["kernelCorrection"] = 2  # the MLS switch is accepted
+    assert build_scene(SimConfiger(config=cfg3)).kernel_corr == 2
     cfg4 = json.loads(json.dumps(cfg))
     cfg4["Blocks"][0]["size"] = [9.0, 0.1, 0.1]
     with pytest.raises(AssertionError):

@@ -42,7 +42,7 @@ struct SceneDev {
   float g, rs2, h1, kW, kC, eps;
   float kG, kGc;  // gradient factors: -5 kW h1^2 (Wendland C2), kC h1^2 (cubic)
   float rs2c;     // stencil-cell culling threshold 1.0001 rs2
-  int kcorr;      // kernelCorrection: 1 = CSPM (corrected kernel gradients)
+  int kcorr;      // kernelCorrection: 1 CSPM, 2 MLS switch (zero corrected gradient)
   int rep;        // add the repulsive-particle force (boundary 3 / 4; WC, muI)
   float rep_k, rep_d, rep_h;  // 0.01 c^2, particle diameter, 0.75 h
   float dt, half_dt, dt6, rho0, visc_c1, visc_c2, grav[3];
@@ -393,27 +393,44 @@ __device__ __forceinline__ size_t nl_at(int k) {
   return ((size_t)(k >> 4) << 10) + (size_t)(k & 15);
 }
 
+// Per-thread table of the list replay (one 9 KiB LDS array per kernel that
+// replays lists, shared by all its replays: a thread only ever touches its
+// own column).
+constexpr int kMaxBlock = 256;  // largest threads per block of a pair kernel
+__device__ __forceinline__ float* shift_table() {
+  __shared__ float tab[9 * kMaxBlock];
+  return tab;
+}
+
 template <int DIM, class Fn>
 __device__ __forceinline__ void for_list(const SceneDev& S, const uint32_t* __restrict__ nl,
                                          int n, int nl_cap, const float4* __restrict__ pos, int i,
                                          float oix, float oiy, float oiz, Fn&& fn) {
   const uint32_t* __restrict__ li = nl + nl_base(i, nl_cap);
   // shifted centres oi + (c_i - c_j) g for c_j - c_i = -1, 0, +1 (exact
-  // products, rounded once: the same values the walk forms)
-  const float bx0 = oix + S.g, bx2 = oix + (-1.0f) * S.g;
-  const float by0 = oiy + S.g, by2 = oiy + (-1.0f) * S.g;
-  const float bz0 = oiz + S.g, bz2 = oiz + (-1.0f) * S.g;
-  auto pick = [](int c, float b0, float b1, float b2) {
-    float b = c == 0 ? b0 : b1;  // two selects, no branch
-    return c == 2 ? b2 : b;
-  };
+  // products, rounded once: the same values the walk forms), kept in LDS as
+  // row 3 * axis + code digit, column = thread, so that a pair looks its
+  // three up by address (conflict-free: every lane reads its own column)
+  // instead of by four compares and selects per axis
+  float* __restrict__ tab = shift_table();
+  const int t = (int)threadIdx.x;
+  tab[0 * kMaxBlock + t] = oix + S.g;
+  tab[1 * kMaxBlock + t] = oix;
+  tab[2 * kMaxBlock + t] = oix + (-1.0f) * S.g;
+  tab[3 * kMaxBlock + t] = oiy + S.g;
+  tab[4 * kMaxBlock + t] = oiy;
+  tab[5 * kMaxBlock + t] = oiy + (-1.0f) * S.g;
+  if (DIM == 3) {
+    tab[6 * kMaxBlock + t] = oiz + S.g;
+    tab[7 * kMaxBlock + t] = oiz;
+    tab[8 * kMaxBlock + t] = oiz + (-1.0f) * S.g;
+  }
   auto pair = [&](uint32_t e, const float4& pj) {
     const int j = (int)(e >> kNlShift);
     const int code = (int)(e & 63u);
-    const int cx = code >> 4, cy = (code >> 2) & 3, cz = code & 3;
-    const float x = pick(cx, bx0, oix, bx2) - pj.x;
-    const float y = pick(cy, by0, oiy, by2) - pj.y;
-    const float z = DIM == 3 ? pick(cz, bz0, oiz, bz2) - pj.z : 0.0f;
+    const float x = tab[(int)((e >> 4) & 3u) * kMaxBlock + t] - pj.x;
+    const float y = tab[(3 + (int)((e >> 2) & 3u)) * kMaxBlock + t] - pj.y;
+    const float z = DIM == 3 ? tab[(6 + (int)(e & 3u)) * kMaxBlock + t] - pj.z : 0.0f;
     const float r2 = fmaf(z, z, fmaf(y, y, x * x));
     fn(j, x, y, z, r2, pj.w, code);
   };

@@ -447,7 +447,7 @@ StepFlags step_flags(tsph_engine* E) {
     first = false;
     f.xsph = f.xsph || E->scenes[s].xsph;
     f.collide = f.collide || E->scenes[s].boundary == 1;
-    f.kcorr = f.kcorr || E->scenes[s].kcorr == 1;
+    f.kcorr = f.kcorr || E->scenes[s].kcorr != 0;
   }
   return f;
 }
@@ -522,8 +522,8 @@ int validate_desc(const tsph_engine* E, const tsph_scene_desc* d) {
   if (d->dim != E->dim) return fail(TSPH_ERR_INVALID_ARG, "scene dimension differs from the engine's");
   if (d->boundary < 0 || d->boundary > 4) return fail(TSPH_ERR_INVALID_ARG, "boundary must be 0..4");
   if (d->kernel != 0 && d->kernel != 1) return fail(TSPH_ERR_INVALID_ARG, "kernel must be 0 or 1");
-  if (d->kernel_corr != 0 && d->kernel_corr != 1)
-    return fail(TSPH_ERR_UNSUPPORTED, "kernelCorrection 2 (MLS) is not built");
+  if (d->kernel_corr < 0 || d->kernel_corr > 2)
+    return fail(TSPH_ERR_INVALID_ARG, "kernelCorrection must be 0, 1 or 2");
   if (d->time_integration != 1 && d->time_integration != 2 && d->time_integration != 4)
     return fail(TSPH_ERR_UNSUPPORTED, "timeIntegration must be 1, 2 or 4");
   if (d->n_particles < 0) return fail(TSPH_ERR_INVALID_ARG, "negative particle count");

@@ -597,17 +597,22 @@ __global__ __launch_bounds__(256) void k_shepard_list(KArgs a, int skip_fluid) {
 // calc_CSPM_L (solver_sph_base.py:400-423) for scenes with
 // kernelCorrection 1: L = sum_{j same type} m_V_j (x_j - x_i) (x) grad W_ij
 // over a flow particle's neighbours, stored inverted; every other particle
-// stores I.
+// stores I.  With kernelCorrection 2 (the MLS switch) calc_kernel_corr
+// computes no matrix (:363-368) and kernel_deriv_corr's branch for it is
+// empty, so the corrected gradient is the zero vector (:375-384): the
+// stored matrix is 0 (the untouched field), and the passes that apply it
+// get a zero gradient.
 template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_cspm_L(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   const SceneDev S = a.scenes[s];
-  if (S.kcorr != 1) return;
-  float M[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+  if (S.kcorr == 0) return;
+  const float one = S.kcorr == 1 ? 1.f : 0.f;
+  float M[9] = {one, 0.f, 0.f, 0.f, one, 0.f, 0.f, 0.f, one};
   const int8_t* __restrict__ type = a.cur.type;
   const int ti = type[p];
-  if (is_flow(ti)) {
+  if (S.kcorr == 1 && is_flow(ti)) {
     const Centre C = centre_of(a, S, p);
     float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
@@ -790,7 +795,7 @@ __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
     return;
   }
   const SceneDev S = a.scenes[s];
-  if (S.kcorr == 1) {
+  if (S.kcorr != 0) {
     if (S.kernel == 1) wc_fluid_one<DIM, NL, SHEP, 1, true>(a, S, p);
     else wc_fluid_one<DIM, NL, SHEP, 0, true>(a, S, p);
   } else {
@@ -815,7 +820,7 @@ __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
   const Centre C = centre_of(a, S, p);
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float4 vi = vtmp[p];
-  const bool kc = S.kcorr == 1;
+  const bool kc = S.kcorr != 0;
   float M[9];
   if (kc) corr_load(a.w.Lc, a.stride, p, M);
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
@@ -961,7 +966,7 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float dv[3] = {0.f, 0.f, 0.f};
-  const bool kc = S.kcorr == 1;
+  const bool kc = S.kcorr != 0;
   float M[9];
   if (kc) corr_load(a.w.Lc, a.stride, p, M);
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
@@ -1205,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + p] / ri2;
   float L[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   float dd = 0.f, dv[3] = {0.f, 0.f, 0.f};
-  const bool kc = S.kcorr == 1;
+  const bool kc = S.kcorr != 0;
   float M[9];
   if (kc) corr_load(a.w.Lc, a.stride, p, M);
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,

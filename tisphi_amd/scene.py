@@ -164,9 +164,10 @@ def build_scene(cfg, generate=True) -> Scene:
         raise NotImplementedError(f"boundary {boundary} is not built")
     kernel = cfg.get_cfg("kernel")
     kcorr = cfg.get_cfg("kernelCorrection")
-    if kcorr not in (0, 1):
-        # MLS (2): the reference's kernel_deriv_corr has no branch body for it
-        raise NotImplementedError("kernelCorrection 2 (MLS) is not built")
+    if kcorr not in (0, 1, 2):
+        # 2 is the reference's MLS switch: kernel_deriv_corr's branch for it
+        # is empty, so every corrected gradient is the zero vector
+        raise NotImplementedError(f"kernelCorrection {kcorr} is not built")
     ti_flag = cfg.get_cfg("timeIntegration")
     if ti_flag not in (1, 2, 4):
         raise NotImplementedError(f"timeIntegration {ti_flag} is not built")
