@@ -323,13 +323,14 @@ __device__ __forceinline__ void neighbours(const SceneDev& S, const KArgs& a, in
 struct ListAppender {
   uint32_t* __restrict__ li;
   int cap, n = 0;
+  // the last three entries, oldest first (a shift register: no per-lane
+  // compare picks the slot)
   uint32_t b0 = 0, b1 = 0, b2 = 0;
   __device__ __forceinline__ void add(uint32_t e) {
-    const int r = n & 3;
-    b0 = r == 0 ? e : b0;
-    b1 = r == 1 ? e : b1;
-    b2 = r == 2 ? e : b2;
-    if (r == 3 && n < cap) *reinterpret_cast<uint4*>(li + nl_at(n - 3)) = make_uint4(b0, b1, b2, e);
+    if ((n & 3) == 3 && n < cap) *reinterpret_cast<uint4*>(li + nl_at(n - 3)) = make_uint4(b0, b1, b2, e);
+    b0 = b1;
+    b1 = b2;
+    b2 = e;
     ++n;
   }
   // the last 1-3 entries
@@ -337,9 +338,9 @@ struct ListAppender {
     const int r = n & 3;
     if (n <= cap && r > 0) {
       const int k = n - r;
-      li[nl_at(k)] = b0;
-      if (r > 1) li[nl_at(k + 1)] = b1;
-      if (r > 2) li[nl_at(k + 2)] = b2;
+      if (r == 3) li[nl_at(k)] = b0;
+      if (r >= 2) li[nl_at(n - 2)] = b1;
+      li[nl_at(n - 1)] = b2;
     }
     return n;
   }
@@ -363,6 +364,29 @@ __device__ __forceinline__ void neighbours_k(const SceneDev& S, const KArgs& a, 
                                         int code) { fn(KTag<0>{}, j, x, y, z, r2, mVj, code); });
 }
 
+// Whether any of the 3^DIM stencil cells around (cx, cy, cz) holds a flow
+// particle (per-cell flags set by the last sort).  Stops at the first one:
+// cheaper than loading all flags together (measured).
+template <int DIM>
+__device__ __forceinline__ bool flow_in_reach(const SceneDev& S, const uint8_t* __restrict__ cflow,
+                                              int cx, int cy, int cz) {
+  const int gx = S.gn[0], gy = S.gn[1], gz = S.gn[2];
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int nx = cx + dx;
+    if (nx < 0 || nx >= gx) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int ny = cy + dy;
+      if (ny < 0 || ny >= gy) continue;
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+        const int nz = cz + dz;
+        if (nz < 0 || nz >= gz) continue;
+        if (cflow[S.cbase + (nx * gy + ny) * gz + nz]) return true;
+      }
+    }
+  }
+  return false;
+}
+
 // Neighbour-list build: one stencil walk per particle per step, recording
 // (j << 6 | stencil code) for every neighbour; boundary particles keep only
 // their flow neighbours (the only ones their passes read).  The candidate
@@ -378,30 +402,11 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
   const bool bdy = is_bdy(type[p]);
   const int cap = a.nl_cap;
   uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
-  if (bdy && use_cflow) {
-    // A boundary particle only records flow neighbours: none can exist if
-    // no stencil cell holds a flow particle (flags set by the last sort).
-    bool any = false;
-    for (int dx = -1; dx <= 1 && !any; ++dx) {
-      const int nx = C.cx + dx;
-      if (nx < 0 || nx >= S.gn[0]) continue;
-      for (int dy = -1; dy <= 1 && !any; ++dy) {
-        const int ny = C.cy + dy;
-        if (ny < 0 || ny >= S.gn[1]) continue;
-        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
-          const int nz = C.cz + dz;
-          if (nz < 0 || nz >= S.gn[2]) continue;
-          if (a.w.cflow[S.cbase + (nx * S.gn[1] + ny) * S.gn[2] + nz]) {
-            any = true;
-            break;
-          }
-        }
-      }
-    }
-    if (!any) {
-      a.w.nlc[p] = 0;
-      return;
-    }
+  // A boundary particle only records flow neighbours: none can exist if
+  // no stencil cell holds a flow particle.
+  if (bdy && use_cflow && !flow_in_reach<DIM>(S, a.w.cflow, C.cx, C.cy, C.cz)) {
+    a.w.nlc[p] = 0;
+    return;
   }
   ListAppender ap{li, cap};
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
@@ -472,29 +477,10 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
     P = pos[p];
     bdy = is_bdy(type[p]);
   }
-  if (act && bdy && use_cflow) {
+  if (act && bdy && use_cflow && !flow_in_reach<DIM>(S, a.w.cflow, cx, cy, cz)) {
     // as in k_nl_build: no flow cell in reach, no neighbours to record
-    bool any = false;
-    for (int dx = -1; dx <= 1 && !any; ++dx) {
-      const int nx = cx + dx;
-      if (nx < 0 || nx >= gx) continue;
-      for (int dy = -1; dy <= 1 && !any; ++dy) {
-        const int ny = cy + dy;
-        if (ny < 0 || ny >= gy) continue;
-        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
-          const int nz = cz + dz;
-          if (nz < 0 || nz >= gz) continue;
-          if (a.w.cflow[S.cbase + (nx * gy + ny) * gz + nz]) {
-            any = true;
-            break;
-          }
-        }
-      }
-    }
-    if (!any) {
-      a.w.nlc[p] = 0;
-      act = false;
-    }
+    a.w.nlc[p] = 0;
+    act = false;
   }
   uint32_t* __restrict__ li = a.w.nl + nl_base(act ? p : t.y, cap);
   ListAppender ap{li, cap};
@@ -540,7 +526,13 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
       const bool staged = cnt <= kStage;  // wave-uniform
       if (staged) {
         __syncthreads();  // the previous row offset's reads are done
-        for (int q = lane; q < cnt; q += 64) sp[q] = pos[jlo + q];
+        // w = 1 for a flow particle (the only neighbours a boundary lane
+        // records), so the filter needs no global load per hit
+        for (int q = lane; q < cnt; q += 64) {
+          float4 v = pos[jlo + q];
+          v.w = is_flow(type[jlo + q]) ? 1.0f : 0.0f;
+          sp[q] = v;
+        }
         __syncthreads();
       }
 #pragma unroll 1
@@ -567,7 +559,24 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
           if (r2 < S.rs2 && j != p) rec(j, x, y, z, r2, pj.w, code);
         };
         if (staged) {
-          for (int32_t j = jb; j < je; ++j) cand(j, sp[j - jlo]);
+          // One induction variable (the LDS slot); the particle index is
+          // formed only on a hit, from an opaque copy of the slot, so the
+          // compiler keeps no per-candidate index, entry or type pointer.
+          // p is a candidate of the own cell only (wave-uniform offset 0).
+          const int32_t skip = (dx == 0 && dy == 0 && dz == 0) ? p - jlo : -1;
+#pragma unroll 1
+          for (int32_t q = jb - jlo; q < je - jlo; ++q) {
+            const float4 pj = sp[q];
+            const float x = bx - pj.x;
+            const float y = by - pj.y;
+            const float z = DIM == 3 ? bz - pj.z : 0.0f;
+            const float r2 = fmaf(z, z, fmaf(y, y, x * x));
+            if (r2 < S.rs2 && q != skip && (!bdy || pj.w != 0.0f)) {
+              int qq = q;
+              asm volatile("" : "+v"(qq));
+              ap.add(nl_entry(jlo + qq, code));
+            }
+          }
         } else {
           for (int32_t j = jb; j < je; ++j) cand(j, pos[j]);
         }
