@@ -130,7 +130,14 @@ enum {
   TSPH_F_CSPM_L = 25,       /* f64 x9, download only: inverse CSPM_L
                                (row-major 3x3; kernelCorrection 1; zero
                                with 2, whose corrected gradient is 0)       */
-  TSPH_F_COUNT = 26
+  /* colour feed (tsph_assign_value_color / tsph_set_color) */
+  TSPH_F_VAL = 26,          /* f64, download only: colour value `val`        */
+  TSPH_F_COLOR = 27,        /* f64 x3, download only: colour (stored as f32) */
+  TSPH_F_X0 = 28,           /* f64 x3 initial position, indexed by id0 (the
+                               scene's original particle order)              */
+  TSPH_F_COLOR0 = 29,       /* f64 x3 initial colour (stored as f32),
+                               indexed by id0                                */
+  TSPH_F_COUNT = 30
 };
 
 /* ---- single passes (test hooks; tsph_step runs the fused sequence) ----- */
@@ -221,6 +228,32 @@ int tsph_set_neighbor_list(void* engine, int enable);
  * out[2] kinetic energy   = sum 1/2 m |v|^2,
  * out[3] mass             = sum m   (flow particles). */
 int tsph_globals(void* engine, int slot, double out[4]);
+
+/* ---- colour feed ---------------------------------------------------------
+ * The reference's per-frame visual pipeline (eng/ui_sim.py:284-286) without
+ * the window, on the device, in the same three calls.  Values are FP64 as
+ * the reference's fields (default_fp=f64, run_simulation.py:23) formed from
+ * this engine's state; colours are f32 as its colour field.
+ *
+ * solver.assign_value_color (eng/solver_sph_base.py:721-789): `val` of the
+ * colorTitle for every real particle, and for dummy particles when
+ * show_bdy; the others hold 0 (the value the reference never overwrites).
+ * Title 44 (distance from the initial position) reads TSPH_F_X0; title 101
+ * (ti.random) is a counter-based uniform in [0, 1) of (id0, seed), which no
+ * check can pin to the reference's generator; unknown titles give 0. */
+int tsph_assign_value_color(void* engine, int slot, int color_title, int show_bdy,
+                            uint32_t seed);
+/* ps.v_maxmin (eng/particle_system.py:387-397): max / min of `val` over flow
+ * particles (-inf / +inf with none), then the givenMax / givenMin / fixMax /
+ * fixMin rule; the pair is kept for tsph_set_color and returned. */
+int tsph_v_maxmin(void* engine, int slot, double given_max, double given_min, int fix_max,
+                  int fix_min, double* out_vmax, double* out_vmin);
+/* ps.set_color (eng/particle_system.py:399-407): the jet colour
+ * (eng/colormap.py:8-48) of (val - vmin) / (vmax - vmin) for the particles
+ * of colorGroup (0 flow, 1 real, 2 real and, with show_bdy, dummy); every
+ * other particle gets its TSPH_F_COLOR0 colour (the reference leaves its
+ * initial colour in place). */
+int tsph_set_color(void* engine, int slot, int color_group, int show_bdy);
 
 /* ---- status / timing ---------------------------------------------------- */
 /* Device status word: out[0] escaped particles, out[1] NaN particles

@@ -24,7 +24,8 @@ FIELDS = dict(pos=(0, "f8", 3), vel=(1, "f8", 3), density=(2, "f8", 1), mass=(3,
               flag_retmap=(16, "i4", 1), cspm_f=(17, "f8", 1), d_density=(18, "f8", 1),
               d_vel=(19, "f8", 3), density_tmp=(20, "f8", 1), v_tmp=(21, "f8", 3),
               cell=(22, "i4", 1), offset=(23, "f8", 3), cell_start=(24, "i4", 1),
-              cspm_L=(25, "f8", 9))
+              cspm_L=(25, "f8", 9), val=(26, "f8", 1), color=(27, "f8", 3), x0=(28, "f8", 3),
+              color0=(29, "f8", 3))
 PASSES = dict(sort=0, shepard=1, init_tmp=2, one_step=3, lf_half=4, lf_full=5,
               advect_pos=6, advect_something=7, step=8, se=9, enforce_boundary=10)
 
@@ -33,7 +34,7 @@ EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_eng
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
            "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list", "tsph_globals",
            "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats", "tsph_create",
-           "tsph_destroy"]
+           "tsph_destroy", "tsph_assign_value_color", "tsph_v_maxmin", "tsph_set_color"]
 
 
 class SceneDesc(ctypes.Structure):
@@ -117,6 +118,9 @@ def load():
         "tsph_neighbor_stats": ([vp, i32, vp, vp, vp], i32),
         "tsph_create": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(vp)], i32),
         "tsph_destroy": ([vp], i32),
+        "tsph_assign_value_color": ([vp, i32, i32, i32, ctypes.c_uint32], i32),
+        "tsph_v_maxmin": ([vp, i32, ctypes.c_double, ctypes.c_double, i32, i32, dp, dp], i32),
+        "tsph_set_color": ([vp, i32, i32, i32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -251,6 +255,21 @@ class Engine:
         check(load().tsph_neighbor_stats(self.h, int(slot), c.ctypes.data, s1.ctypes.data,
                                          s2.ctypes.data))
         return c, s1, s2
+
+    # colour feed (solver.assign_value_color, ps.v_maxmin, ps.set_color)
+    def assign_value_color(self, slot, color_title, show_bdy=False, seed=0):
+        check(load().tsph_assign_value_color(self.h, int(slot), int(color_title),
+                                             1 if show_bdy else 0, int(seed) & 0xFFFFFFFF))
+
+    def v_maxmin(self, slot, givenmax=-1.0, givenmin=-1.0, fixmax=0, fixmin=0):
+        hi, lo = ctypes.c_double(), ctypes.c_double()
+        check(load().tsph_v_maxmin(self.h, int(slot), float(givenmax), float(givenmin),
+                                   1 if fixmax else 0, 1 if fixmin else 0, ctypes.byref(hi),
+                                   ctypes.byref(lo)))
+        return hi.value, lo.value
+
+    def set_color(self, slot, color_group, show_bdy=False):
+        check(load().tsph_set_color(self.h, int(slot), int(color_group), 1 if show_bdy else 0))
 
     def kernel_time(self, name):
         ms = ctypes.c_double()

@@ -97,8 +97,14 @@ struct PWork {
   unsigned long long* status;  // per slot: escaped, NaN
   double* gpart;     // globals partials per tile (4 each)
   double* gout;      // globals per slot (4 each)
-  uint32_t* nl;      // neighbour lists, entry k of particle p at [k*cap + p]
+  uint32_t* nl;      // neighbour lists, blocked per 64-particle tile (nl_base / nl_at)
   int32_t* nlc;      // neighbour count per particle (> nl_cap: walk instead)
+  // colour feed (allocated on first use)
+  double* val;       // colour value per particle (current order)
+  float4* rgb;       // colour per particle (current order)
+  double* x0;        // 3 x cap initial position, indexed by scene base + id0
+  float4* c0;        // initial colour, indexed by scene base + id0
+  double* crange;    // per slot: vmax, vmin of the last v_maxmin
 };
 
 struct KArgs {

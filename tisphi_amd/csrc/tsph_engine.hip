@@ -203,7 +203,8 @@ int alloc_work(tsph_engine* E) {
 void free_work(PWork& w) {
   void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.vp, w.Lc, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
                   w.dsig_d, w.dseq, w.dseqp, w.xs, w.drho_rk, w.dv_rk, w.dsig_rk, w.count,
-                  w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc, w.cflow};
+                  w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc, w.cflow,
+                  w.val, w.rgb, w.x0, w.c0, w.crange};
   for (void* p : ptrs)
     if (p) hipFree(p);
   w = PWork{};
@@ -216,6 +217,21 @@ void invalidate_graph(tsph_engine* E) {
       g = nullptr;
     }
   E->graph_valid = false;
+}
+
+// The colour feed's arrays, on first use (an engine that never colours
+// does not hold them).
+int ensure_colour(tsph_engine* E) {
+  PWork& w = E->w;
+  if (w.val) return TSPH_OK;
+  size_t& b = E->bytes;
+  const size_t n = E->cap;
+  int r;
+  if ((r = dalloc(&w.val, n, b)) || (r = dalloc(&w.rgb, n, b)) || (r = dalloc(&w.x0, 3 * n, b)) ||
+      (r = dalloc(&w.c0, n, b)) || (r = dalloc(&w.crange, 2 * (size_t)E->max_scenes, b)))
+    return r;
+  invalidate_graph(E);  // the kernel arguments captured in the graphs changed
+  return TSPH_OK;
 }
 
 // ----------------------------------------------------------------- launching
@@ -617,6 +633,7 @@ struct FieldInfo {
 FieldInfo field_info(int f) {
   switch (f) {
     case TSPH_F_POS: case TSPH_F_VEL: case TSPH_F_D_VEL: case TSPH_F_V_TMP: case TSPH_F_OFFSET:
+    case TSPH_F_COLOR: case TSPH_F_X0: case TSPH_F_COLOR0:
       return {3, false};
     case TSPH_F_STRESS: case TSPH_F_STRESS_TMP: case TSPH_F_D_STRESS:
       return {6, false};
@@ -861,6 +878,30 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
     case TSPH_F_D_STRAIN_EQU_P:
       if (!w.dseqp) { if (!up) std::memset(hd, 0, n * 8); return TSPH_OK; }
       return d1(w.dseqp);
+    case TSPH_F_VAL:
+    case TSPH_F_COLOR:
+    case TSPH_F_X0:
+    case TSPH_F_COLOR0: {
+      if (up && (field == TSPH_F_VAL || field == TSPH_F_COLOR))
+        return fail(TSPH_ERR_INVALID_ARG, "val and color are download-only");
+      if ((r = ensure_colour(E))) return r;
+      if (field == TSPH_F_VAL) return d1(w.val);
+      if (field == TSPH_F_X0) {
+        for (int c = 0; c < 3; ++c) {
+          std::vector<double> v;
+          if (up) {
+            v.resize(n);
+            for (int64_t i = 0; i < n; ++i) v[i] = hd[3 * i + c];
+            if ((r = h2d(E, w.x0 + (size_t)c * st, off, v))) return r;
+          } else {
+            if ((r = d2h(E, (const double*)(w.x0 + (size_t)c * st), off, n, v))) return r;
+            for (int64_t i = 0; i < n; ++i) hd[3 * i + c] = v[i];
+          }
+        }
+        return TSPH_OK;
+      }
+      return f4(field == TSPH_F_COLOR ? w.rgb : w.c0, 0, 3, false);
+    }
   }
   return fail(TSPH_ERR_INVALID_ARG, "unhandled field");
 }
@@ -1159,6 +1200,48 @@ int tsph_globals(void* engine, int slot, double out[4]) {
   HIP_TRY(hipMemcpyAsync(out, E->w.gout + 4 * slot, 32, hipMemcpyDeviceToHost, E->stream));
   HIP_TRY(hipStreamSynchronize(E->stream));
   return TSPH_OK;
+}
+
+int tsph_assign_value_color(void* engine, int slot, int color_title, int show_bdy, uint32_t seed) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!slot_ok(E, slot)) return fail(TSPH_ERR_INVALID_ARG, "bad slot");
+  int r = ensure_colour(E);
+  if (r) return r;
+  Launcher L{E};
+  L.go("color", k_color_val, E->nblocks_tiles(256), 256, E->args(), slot, color_title,
+       show_bdy ? 1 : 0, seed);
+  return L.status;
+}
+
+int tsph_v_maxmin(void* engine, int slot, double given_max, double given_min, int fix_max,
+                  int fix_min, double* out_vmax, double* out_vmin) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!slot_ok(E, slot) || !out_vmax || !out_vmin) return fail(TSPH_ERR_INVALID_ARG, "bad argument");
+  int r = ensure_colour(E);
+  if (r) return r;
+  Launcher L{E};
+  const KArgs a = E->args();
+  L.go("color", k_color_range_tile, E->nblocks_tiles(256), 256, a, slot);
+  L.go("color", k_color_range_scene, 1, 256, a, slot, given_max, given_min, fix_max ? 1 : 0,
+       fix_min ? 1 : 0);
+  if (L.status) return L.status;
+  double v[2];
+  HIP_TRY(hipMemcpyAsync(v, E->w.crange + 2 * slot, 16, hipMemcpyDeviceToHost, E->stream));
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  *out_vmax = v[0];
+  *out_vmin = v[1];
+  return TSPH_OK;
+}
+
+int tsph_set_color(void* engine, int slot, int color_group, int show_bdy) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!slot_ok(E, slot)) return fail(TSPH_ERR_INVALID_ARG, "bad slot");
+  int r = ensure_colour(E);
+  if (r) return r;
+  Launcher L{E};
+  L.go("color", k_set_color, E->nblocks_tiles(256), 256, E->args(), slot, color_group,
+       show_bdy ? 1 : 0);
+  return L.status;
 }
 
 int tsph_status(void* engine, int slot, int64_t out[2]) {

@@ -1692,4 +1692,176 @@ __global__ __launch_bounds__(256) void k_globals_scene(KArgs a, int nslots) {
   }
 }
 
+// ============================================================== colour feed
+// assign_value_color (solver_sph_base.py:721-789): the colour value of one
+// colorTitle in FP64 (the reference's default_fp), for real particles and,
+// with show_bdy, dummy particles; 0 for the rest.
+__device__ __forceinline__ double norm3(double x, double y, double z) {
+  return sqrt(x * x + y * y + z * z);
+}
+// SplitMix64 (Steele, Lea & Flood, OOPSLA 2014) of (seed, id0) -> [0, 1),
+// standing in for ti.random() of title 101.
+__device__ __forceinline__ double uniform01(uint32_t seed, int32_t id) {
+  uint64_t z = ((uint64_t)seed << 32 | (uint32_t)id) + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+__global__ __launch_bounds__(256) void k_color_val(KArgs a, int slot, int title, int show_bdy,
+                                                   uint32_t seed) {
+  int s, p;
+  if (!tile_thread(a, s, p) || s != slot) return;
+  const SceneDev S = a.scenes[s];
+  const int t = a.cur.type[p];
+  double v = 0.0;
+  if (is_real(t) || (show_bdy && t == kDummy)) {
+    const int st = a.stride;
+    auto stress = [&](int c) -> double {
+      if (a.cur.sig_d) return a.cur.sig_d[c * st + p];
+      if (a.cur.sig_f) return (double)a.cur.sig_f[c * st + p];
+      return 0.0;  // WC: the fluid's stress field stays zero
+    };
+    auto position = [&](double x[3]) {
+      int cc[3];
+      decode_cell(S, a.cur.cell[p], cc[0], cc[1], cc[2]);
+      const float4 P = a.cur.pos[p];
+      x[0] = S.vds[0] + (double)cc[0] * S.grid + (double)P.x;
+      x[1] = S.vds[1] + (double)cc[1] * S.grid + (double)P.y;
+      x[2] = S.dim == 3 ? S.vds[2] + (double)cc[2] * S.grid + (double)P.z : 0.0;
+    };
+    const float4 vel = a.cur.vel[p];
+    const float4 dv = a.w.dv[p];
+    double x[3];
+    switch (title) {
+      case 1: v = (double)a.cur.id0[p]; break;
+      case 2: v = (double)vel.w; break;
+      case 21: v = (double)a.w.drho[p]; break;
+      case 3: v = norm3(vel.x, vel.y, vel.z); break;
+      case 31: v = (double)vel.x; break;
+      case 32: v = -(double)vel.y; break;
+      case 33: v = (double)vel.z; break;
+      case 34: v = norm3(dv.x, dv.y, dv.z); break;
+      case 35: v = (double)dv.x; break;
+      case 36: v = (double)dv.y; break;
+      case 37: v = (double)dv.z; break;
+      case 4: position(x); v = norm3(x[0], x[1], x[2]); break;
+      case 41: position(x); v = x[0]; break;
+      case 42: position(x); v = x[1]; break;
+      case 43: position(x); v = x[2]; break;
+      case 44: {
+        position(x);
+        const size_t k = (size_t)S.pbase + (size_t)a.cur.id0[p];
+        v = norm3(x[0] - a.w.x0[k], x[1] - a.w.x0[st + k], x[2] - a.w.x0[2 * (size_t)st + k]);
+        break;
+      }
+      case 51: v = stress(0); break;
+      case 52: v = -stress(1); break;
+      case 53: v = -stress(2); break;
+      case 54: v = stress(3); break;
+      case 55: v = stress(4); break;
+      case 56: v = stress(5); break;
+      case 57: v = -(stress(0) + stress(1) + stress(2)) / 3.0; break;
+      case 61: v = a.cur.seq ? a.cur.seq[p] : 0.0; break;
+      case 62: v = a.cur.seqp ? a.cur.seqp[p] : 0.0; break;
+      case 7: v = (double)a.w.p[p]; break;
+      case 8: v = a.cur.flag ? (double)a.cur.flag[p] : 0.0; break;
+      case 100: v = (double)a.cur.cell[p]; break;
+      case 101: v = uniform01(seed, a.cur.id0[p]); break;
+      case 102: v = S.kcorr == 1 ? (double)a.w.Lc[p] : 0.0; break;
+      case 103: {
+        const float4 vt = a.cur.vtmp[p];
+        v = norm3(vt.x, vt.y, vt.z);
+        break;
+      }
+      default: v = 0.0;
+    }
+  }
+  a.w.val[p] = v;
+}
+
+// v_maxmin (particle_system.py:387-397): max / min of val over flow
+// particles; per-tile partials, then one block per scene, then the
+// givenMax / givenMin / fixMax / fixMin rule.  max / min do not depend on the
+// reduction order.
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__global__ __launch_bounds__(256) void k_color_range_tile(KArgs a, int slot) {
+  const int w = wave_id();
+  const int2 t = a.tiles[w];
+  if (t.x != slot) return;  // wave-uniform
+  const SceneDev S = a.scenes[t.x];
+  const int p = t.y + (int)(threadIdx.x & 63);
+  double hi = -INFINITY, lo = INFINITY;
+  if (p < S.pbase + S.n && is_flow(a.cur.type[p])) hi = lo = a.w.val[p];
+  hi = wave_max(hi);
+  lo = wave_min(lo);
+  if ((threadIdx.x & 63) == 0) {
+    a.w.gpart[4 * (size_t)w] = hi;
+    a.w.gpart[4 * (size_t)w + 1] = lo;
+  }
+}
+__global__ __launch_bounds__(256) void k_color_range_scene(KArgs a, int slot, double given_max,
+                                                           double given_min, int fix_max,
+                                                           int fix_min) {
+  const SceneDev S = a.scenes[slot];
+  double hi = -INFINITY, lo = INFINITY;
+  for (int k = threadIdx.x; k < S.n_tiles; k += blockDim.x) {
+    const double* o = a.w.gpart + 4 * (size_t)(S.tile_first + k);
+    hi = fmax(hi, o[0]);
+    lo = fmin(lo, o[1]);
+  }
+  __shared__ double red[2][256 / 64];
+  hi = wave_max(hi);
+  lo = wave_min(lo);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = hi;
+    red[1][wid] = lo;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      hi = fmax(hi, red[0][k]);
+      lo = fmin(lo, red[1][k]);
+    }
+    // vmax if givenmax == -1 or (vmax < givenmax and not fixmax) else givenmax
+    a.w.crange[2 * slot] = (given_max == -1.0 || (hi < given_max && !fix_max)) ? hi : given_max;
+    a.w.crange[2 * slot + 1] = (given_min == -1.0 || (lo > given_min && !fix_min)) ? lo : given_min;
+  }
+}
+
+// set_color (particle_system.py:399-407) with the jet map of colormap.py:
+// map(x) = clamp((w - |clamp(x) - c|) / w * h), w = wl below c, wr above,
+// clamp to [0, 1]; (h, wl, wr, c) = (1.5, .37, .37, .75 / .5 / .25) for
+// R / G / B.  FP64 as the reference's Python constants, stored as f32.
+__device__ __forceinline__ double clamp01(double x) { return fmax(0.0, fmin(1.0, x)); }
+__device__ __forceinline__ double jet_channel(double x, double c) {
+  const double h = 1.5, wl = 0.37, wr = 0.37;
+  const double w = x < c ? wl : wr;
+  return clamp01((w - fabs(clamp01(x) - c)) / w * h);
+}
+__global__ __launch_bounds__(256) void k_set_color(KArgs a, int slot, int group, int show_bdy) {
+  int s, p;
+  if (!tile_thread(a, s, p) || s != slot) return;
+  const SceneDev S = a.scenes[s];
+  const int t = a.cur.type[p];
+  const bool member = (group == 0 && is_flow(t)) || (group == 1 && is_real(t)) ||
+                      (group == 2 && (is_real(t) || (show_bdy && t == kDummy)));
+  float4 c;
+  if (member) {
+    const double vmax = a.w.crange[2 * slot], vmin = a.w.crange[2 * slot + 1];
+    const double vrange1 = 1.0 / (vmax - vmin);
+    const double x = (a.w.val[p] - vmin) * vrange1;
+    c = make_float4((float)jet_channel(x, 0.75), (float)jet_channel(x, 0.5),
+                    (float)jet_channel(x, 0.25), 0.0f);
+  } else {
+    c = a.w.c0[(size_t)S.pbase + (size_t)a.cur.id0[p]];
+  }
+  a.w.rgb[p] = c;
+}
+
 }  // namespace tsph

@@ -72,7 +72,7 @@ class _ParticleAccessor:
             "d_strain_equ": "d_strain_equ", "d_strain_equ_p": "d_strain_equ_p",
             "flag_retmap": "flag_retmap", "CSPM_f": "cspm_f", "d_density": "d_density",
             "d_vel": "d_vel", "density_tmp": "density_tmp", "v_tmp": "v_tmp",
-            "grid_ids": "cell"}
+            "grid_ids": "cell", "val": "val", "color": "color"}
     _mat = {"stress": "stress", "stress_tmp": "stress_tmp", "d_stress": "d_stress"}
 
     def __init__(self, ps):
@@ -133,6 +133,8 @@ class ParticleSystem:
             self.object_collection[b["objectId"]] = b
         self.particle_max_num = sc.n
         self._vmax, self._vmin = 0.0, 0.0
+        self._colour_inputs = False
+        self._colour_calls = 0
         self.vmax = ScalarField(lambda: self._vmax)
         self.vmin = ScalarField(lambda: self._vmin)
         self.particle_num = ScalarField(lambda: self.particle_max_num)
@@ -160,17 +162,25 @@ class ParticleSystem:
         return self.scene.mat_id[order]
 
     # ------------------------------------------------------------ read-back
-    def v_maxmin(self, givenmax=-1, givenmin=-1, fixmax=0, fixmin=0, val=None):
-        """Max/min of the colour value over flow particles
-        (particle_system.py:387-397)."""
-        if val is None:
-            val = self.engine.download(self.slot, "pos")[:, 0]
-        t = self.engine.download(self.slot, "type")
-        flow = (t == MAT_FLUID) | (t == MAT_SOIL)
-        vmax = float(np.max(val[flow])) if np.any(flow) else -np.inf
-        vmin = float(np.min(val[flow])) if np.any(flow) else np.inf
-        self._vmax = vmax if givenmax == -1 or (vmax < givenmax and not fixmax) else givenmax
-        self._vmin = vmin if givenmin == -1 or (vmin > givenmin and not fixmin) else givenmin
+    def _upload_colour_inputs(self):
+        """Initial positions (colour title 44) and initial colours (kept by
+        particles outside the colour group), indexed by id0, once."""
+        if not self._colour_inputs:
+            self.engine.upload(self.slot, "x0", self.scene.pos)
+            self.engine.upload(self.slot, "color0", self.scene.color0.astype(np.float64))
+            self._colour_inputs = True
+
+    def v_maxmin(self, givenmax=-1, givenmin=-1, fixmax=0, fixmin=0):
+        """Max/min of the colour value over flow particles, then the
+        given/fix rule (particle_system.py:387-397), on the device."""
+        self._upload_colour_inputs()
+        self._vmax, self._vmin = self.engine.v_maxmin(self.slot, givenmax, givenmin, fixmax, fixmin)
+
+    def set_color(self):
+        """Jet colour of the colour group (particle_system.py:399-407), on
+        the device (``ps.pt.color`` reads it)."""
+        self._upload_colour_inputs()
+        self.engine.set_color(self.slot, self.color_group, self.show_bdy)
 
     def dump(self):
         """(positions, data) dictionaries of particle_system.py:459-545, in

@@ -104,49 +104,12 @@ class SPHBase:
 
     # ----------------------------------------------------------- visuals
     def assign_value_color(self):
-        """Colour value per particle (solver_sph_base.py:721-789), computed
-        on the host from downloaded fields; returns the array."""
-        ps, e, s = self.ps, self.ps.engine, self.ps.slot
-        t = ps.color_title
-        if t in (2,):
-            val = e.download(s, "density")
-        elif t == 1:
-            val = e.download(s, "id0").astype(np.float64)
-        elif t == 21:
-            val = e.download(s, "d_density")
-        elif t in (3, 31, 32, 33):
-            v = e.download(s, "vel")
-            val = {3: np.linalg.norm(v, axis=1), 31: v[:, 0], 32: -v[:, 1], 33: v[:, 2]}[t]
-        elif t in (34, 35, 36, 37):
-            v = e.download(s, "d_vel")
-            val = {34: np.linalg.norm(v, axis=1), 35: v[:, 0], 36: v[:, 1], 37: v[:, 2]}[t]
-        elif t in (4, 41, 42, 43, 44):
-            x = e.download(s, "pos")
-            if t == 44:
-                x0 = ps.scene.pos[e.download(s, "id0")]
-                val = np.linalg.norm(x - x0, axis=1)
-            else:
-                val = {4: np.linalg.norm(x, axis=1), 41: x[:, 0], 42: x[:, 1], 43: x[:, 2]}[t]
-        elif t in (51, 52, 53, 54, 55, 56, 57):
-            st = e.download(s, "stress")
-            val = {51: st[:, 0], 52: -st[:, 1], 53: -st[:, 2], 54: st[:, 3], 55: st[:, 4],
-                   56: st[:, 5], 57: -(st[:, 0] + st[:, 1] + st[:, 2]) / 3.0}[t]
-        elif t == 61:
-            val = e.download(s, "strain_equ")
-        elif t == 62:
-            val = e.download(s, "strain_equ_p")
-        elif t == 7:
-            val = e.download(s, "pressure")
-        elif t == 8:
-            val = e.download(s, "flag_retmap").astype(np.float64)
-        elif t == 100:
-            val = e.download(s, "cell").astype(np.float64)
-        elif t == 103:
-            val = np.linalg.norm(e.download(s, "v_tmp"), axis=1)
-        else:
-            val = np.zeros(ps.particle_max_num)
-        self._val = val
-        return val
+        """Colour value per particle (solver_sph_base.py:721-789), on the
+        device (``ps.pt.val`` reads it)."""
+        ps = self.ps
+        ps._upload_colour_inputs()
+        ps._colour_calls += 1
+        ps.engine.assign_value_color(ps.slot, ps.color_title, ps.show_bdy, seed=ps._colour_calls)
 
     # material helpers used by the concrete solvers
     @staticmethod

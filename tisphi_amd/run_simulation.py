@@ -10,6 +10,9 @@ end the run (``cur_time = dt * count_step``), ``stopAtStep`` /
 ignored, exports follow ``exportEveryTime`` / ``exportEveryRender`` with
 ``exportCSV`` / ``exportVTK`` (PNG frames need the window and are not
 written).  ``--steps`` caps the run for scenes that set neither exit rule.
+As the window loop, the run colours the particles at step 0 and after every
+frame (``assign_color``, eng/ui_sim.py:94, 107, 282-286) when colorTitle > 0,
+on the device: ``case.ps.pt.val`` / ``case.ps.pt.color`` hold the result.
 """
 import argparse
 import json
@@ -20,6 +23,14 @@ import time
 from . import export
 from .eng.configer_builder import SimConfiger
 from .eng.simulation import Simulation
+
+
+def assign_color(case, given_max, given_min, fix_max, fix_min):
+    """eng/ui_sim.py:282-286: value, range and colour of the frame."""
+    if case.ps.color_title > 0:
+        case.solver.assign_value_color()
+        case.ps.v_maxmin(given_max, given_min, fix_max, fix_min)
+        case.ps.set_color()
 
 
 def run(scene_path, max_steps=None, out_dir=None, log=print):
@@ -43,6 +54,8 @@ def run(scene_path, max_steps=None, out_dir=None, log=print):
     dt = case.solver.dt[None]
     count_step = 0
     next_time = every_time
+    colour = (g("givenMax", -1), g("givenMin", -1), g("fixMax", 0), g("fixMin", 0))
+    assign_color(case, *colour)
 
     def do_export(stamp):
         pos, data = case.ps.dump()
@@ -62,6 +75,7 @@ def run(scene_path, max_steps=None, out_dir=None, log=print):
             n = min(n, max_steps - count_step)
         case.solver.steps(n)
         count_step += n
+        assign_color(case, *colour)
         cur_time = dt * count_step
         if judge_path:
             if every_time > 0 and cur_time >= next_time:

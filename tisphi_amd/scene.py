@@ -115,6 +115,7 @@ class Scene:
     mat_type: np.ndarray = None
     obj_id: np.ndarray = None
     mat_id: np.ndarray = None
+    color0: np.ndarray = None  # initial colour per particle (float32 values)
     stress: np.ndarray = None  # (n, 6) xx,yy,zz,xy,yz,zx
 
     @property
@@ -213,10 +214,12 @@ def build_scene(cfg, generate=True) -> Scene:
     if not generate:
         return sc
 
-    pos, vel, rho, mtype, oid, mid = [], [], [], [], [], []
+    pos, vel, rho, mtype, oid, mid, col = [], [], [], [], [], [], []
     lists = (fluids, soils, rigids)
     for b in blocks:
         mat = get_material(mat_index, lists, b["materialId"])
+        # the material colour / 255 as float32 (particle_system.py:159)
+        col.append(np.array([ic / 255 for ic in mat.get("color", [0, 0, 0])], dtype=np.float32))
         p = cube_positions(np.array(b["translation"], dtype=np.float64),
                            np.array(b["size"], dtype=np.float64), geo.dim, geo.particle_diameter)
         n = p.shape[0]
@@ -226,6 +229,7 @@ def build_scene(cfg, generate=True) -> Scene:
         mtype.append(np.full(n, mat["matType"], dtype=np.int32))
         oid.append(np.full(n, b["objectId"], dtype=np.int32))
         mid.append(np.full(n, b["materialId"], dtype=np.int32))
+        col[-1] = np.tile(col[-1], (n, 1))
     if dummy is not None:
         for lo, hi in dummy:
             p = cube_positions(lo, hi - lo, geo.dim, geo.particle_diameter)
@@ -236,6 +240,8 @@ def build_scene(cfg, generate=True) -> Scene:
             mtype.append(np.full(n, MAT_DUMMY, dtype=np.int32))
             oid.append(np.full(n, MAT_DUMMY, dtype=np.int32))
             mid.append(np.zeros(n, dtype=np.int32))
+            # add_boundary colour [153, 153, 255] / 255 (particle_system.py:146)
+            col.append(np.tile((np.array([153, 153, 255]) / 255).astype(np.float32), (n, 1)))
     if rep is not None:
         for lo, hi in rep:
             p = cube_positions(lo, hi - lo, geo.dim, geo.particle_radius)
@@ -246,12 +252,15 @@ def build_scene(cfg, generate=True) -> Scene:
             mtype.append(np.full(n, MAT_REP, dtype=np.int32))
             oid.append(np.full(n, MAT_REP, dtype=np.int32))
             mid.append(np.zeros(n, dtype=np.int32))
+            # [170, 17, 255] / 255 (particle_system.py:148)
+            col.append(np.tile((np.array([170, 17, 255]) / 255).astype(np.float32), (n, 1)))
     sc.pos = np.concatenate(pos) if pos else np.zeros((0, 3))
     sc.vel = np.concatenate(vel) if vel else np.zeros((0, 3))
     sc.density = np.concatenate(rho) if rho else np.zeros(0)
     sc.mat_type = np.concatenate(mtype) if mtype else np.zeros(0, np.int32)
     sc.obj_id = np.concatenate(oid) if oid else np.zeros(0, np.int32)
     sc.mat_id = np.concatenate(mid) if mid else np.zeros(0, np.int32)
+    sc.color0 = np.concatenate(col) if col else np.zeros((0, 3), np.float32)
     sc.m_V = np.full(sc.n, geo.m_V0)
     sc.mass = geo.m_V0 * sc.density
     sc.stress = np.zeros((sc.n, 6))
