@@ -67,7 +67,8 @@ enum { TSPH_METHOD_WC = 1, TSPH_METHOD_MUI = 2, TSPH_METHOD_DP = 3 };
 typedef struct tsph_scene_desc {
   int32_t dim;              /* 2 or 3                                        */
   int32_t method;           /* TSPH_METHOD_*                                 */
-  int32_t boundary;         /* 0 none, 1 collision, 2 dummy (3/4 rejected)   */
+  int32_t boundary;         /* 0 none, 1 collision, 2 dummy, 3 repulsive,
+                               4 dummy + repulsive                           */
   int32_t kernel;           /* 0 cubic spline, 1 Wendland C2                 */
   int32_t kernel_corr;      /* 0 none, 1 CSPM, 2 MLS switch (zero gradient)  */
   int32_t time_integration; /* 1 symplectic Euler, 2 leap-frog, 4 RK4        */
@@ -152,7 +153,9 @@ enum {
   TSPH_PASS_ADVECT_SOMETHING = 7, /* solver-specific post-update            */
   TSPH_PASS_STEP = 8,             /* one full step(), one launch per pass   */
   TSPH_PASS_SE = 9,               /* advect_SE                              */
-  TSPH_PASS_ENFORCE_BOUNDARY = 10 /* enforce_boundary (boundary == 1)       */
+  TSPH_PASS_ENFORCE_BOUNDARY = 10, /* enforce_boundary (boundary == 1, and
+                                     dynamic rigid particles)               */
+  TSPH_PASS_SOLVE_RIGID = 11      /* solve_rigid_body: shape matching       */
 };
 
 /* ---- version / errors --------------------------------------------------- */
@@ -228,6 +231,22 @@ int tsph_set_neighbor_list(void* engine, int enable);
  * out[2] kinetic energy   = sum 1/2 m |v|^2,
  * out[3] mass             = sum m   (flow particles). */
 int tsph_globals(void* engine, int slot, double out[4]);
+
+/* ---- rigid bodies ---------------------------------------------------------
+ * Rigid blocks of a WCSPH scene (TSPH_MAT_RIGID particles; ParticleSystem
+ * init_block, particle_system.py:150-174) are walls to the fluid passes
+ * (dummy extrapolation, solver_sph_wc.py:90-103).  A static block keeps
+ * d_vel = 0 (:125-126).  A dynamic block (isDynamic) is registered here
+ * with its object id and its rest centre of mass (calc_rigid_rest_m,
+ * solver_sph_base.py:467-470, 517-518, the mass-weighted mean of the
+ * initial positions): its particles take g minus the fluid's pair forces
+ * (:105-106, 68-71), move with the integrator, are put back into shape by
+ * solve_rigid_body (solver_sph_base.py:472-510; initial positions from
+ * TSPH_F_X0) and are always kept inside the domain (judge_enforce_bdy,
+ * :532-538).  At most 4 dynamic objects per scene; TSPH_ERR_UNSUPPORTED for
+ * muI / DP engines. */
+int tsph_set_rigid_object(void* engine, int slot, int object_id, const double rest_cm[3]);
+/* (registering a known object id again replaces its rest centre) */
 
 /* ---- colour feed ---------------------------------------------------------
  * The reference's per-frame visual pipeline (eng/ui_sim.py:284-286) without

@@ -21,9 +21,10 @@ FIELDS = dict(pos=(0, "f8", 3), vel=(1, "f8", 3), density=(2, "f8", 1), mass=(3,
               flag_retmap=(16, "i4", 1), cspm_f=(17, "f8", 1), d_density=(18, "f8", 1),
               d_vel=(19, "f8", 3), density_tmp=(20, "f8", 1), v_tmp=(21, "f8", 3),
               cell=(22, "i4", 1), offset=(23, "f8", 3), cell_start=(24, "i4", 1),
-              cspm_L=(25, "f8", 9))
+              cspm_L=(25, "f8", 9), x0=(28, "f8", 3))
 PASSES = dict(sort=0, shepard=1, init_tmp=2, one_step=3, lf_half=4, lf_full=5,
-              advect_pos=6, advect_something=7, step=8, se=9, enforce_boundary=10)
+              advect_pos=6, advect_something=7, step=8, se=9, enforce_boundary=10,
+              solve_rigid=11)
 
 
 class OrcParams(ctypes.Structure):
@@ -91,6 +92,9 @@ def lib():
         L.orc_dp_rate.argtypes = [P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_get_threads.restype = ctypes.c_int
+        L.orc_set_rigid_object.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double)]
+        L.orc_polar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -154,6 +158,11 @@ class Oracle:
     def escaped(self):
         return lib().orc_escaped(self.h)
 
+    def set_rigid_object(self, obj, rest_cm):
+        cm = (ctypes.c_double * 3)(*[float(v) for v in rest_cm])
+        if lib().orc_set_rigid_object(self.h, int(obj), cm) != 0:
+            raise ValueError("at most 4 dynamic rigid objects")
+
 
 def load_scene(oracle, scene):
     """Upload a freshly generated tisphi_amd Scene (generation order)."""
@@ -167,6 +176,18 @@ def load_scene(oracle, scene):
     oracle.set("id0", np.arange(n, dtype=np.int32))
     oracle.set("obj_id", scene.obj_id)
     oracle.set("stress", scene.stress)
+    oracle.set("x0", scene.pos)
+    for r in getattr(scene, "rigid", []):
+        if r["dynamic"]:
+            oracle.set_rigid_object(r["obj"], r["rest_cm"])
+
+
+def polar(A, dim):
+    """The oracle's orthogonal polar factor of a 3x3 matrix (row-major)."""
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
+    R = np.zeros(9)
+    lib().orc_polar(A.ctypes.data, int(dim), R.ctypes.data)
+    return R.reshape(3, 3)
 
 
 def kernel_eval(desc, r, precision=64):

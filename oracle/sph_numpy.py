@@ -16,6 +16,10 @@ formulas against a second reading of the reference:
   muI one_step/advect ... eng/solver_sph_muI.py:171-189, 195-289
   DP one_step/advect .... eng/solver_sph_dp.py:41-118, 171-296
   dummy tasks ........... eng/solver_sph_base.py:647-669, 713-715
+  rigid bodies .......... eng/solver_sph_wc.py:68-71, 90-106, 125-126;
+                          eng/solver_sph_base.py:472-518 (the fluid's pair
+                          force scattered onto rigid neighbours as the
+                          reference writes it; polar factor by SVD)
 Parity-mode decisions D1-D3 (two-phase loops) are applied as in the C++
 oracle.  Only for small scenes (N up to a few thousand).
 """
@@ -35,7 +39,8 @@ def is_bdy(t):
 
 
 class NumpySPH:
-    def __init__(self, desc, pos, vel, density, mass, m_V, mtype, stress6=None):
+    def __init__(self, desc, pos, vel, density, mass, m_V, mtype, stress6=None, obj=None,
+                 rigid=()):
         self.d = dict(desc)
         self.dim = desc["dim"]
         self.x = np.array(pos, dtype=np.float64)
@@ -76,6 +81,15 @@ class NumpySPH:
         self.G = desc["young"] / (2 * (1 + desc["poisson"]))
         self.K = desc["young"] / (3 * (1 - 2 * desc["poisson"]))
         self.sin_psi = math.sin(desc["dilatancy"])
+        # rigid blocks: object ids, initial positions, dynamic objects' rest
+        # centres (calc_rigid_rest_m, solver_sph_base.py:517-518)
+        self.obj = np.zeros(n, dtype=np.int64) if obj is None else np.asarray(obj, dtype=np.int64)
+        self.x0 = self.x.copy()
+        self.rigid = [(r["obj"], np.asarray(r["rest_cm"], dtype=np.float64)) for r in rigid
+                      if r["dynamic"]]
+        self.rig_dyn = np.zeros(n, dtype=bool)
+        for o, _ in self.rigid:
+            self.rig_dyn |= (self.t == RIGID) & (self.obj == o)
 
     # ------------------------------------------------------------- kernels
     def W(self, r):
@@ -165,8 +179,10 @@ class NumpySPH:
         self.sigt[soil] = self.sig[soil]
 
     def _bdy_common(self, i, j, xij, r):
-        """Sums over flow neighbours of boundary particles."""
-        w = np.where(is_bdy(self.t[i]) & is_flow(self.t[j]), self.W(r), 0.0)
+        """Sums over flow neighbours of boundary (and, in WC, rigid)
+        particles."""
+        solid = is_bdy(self.t[i]) | (self.t[i] == RIGID)
+        w = np.where(solid & is_flow(self.t[j]), self.W(r), 0.0)
         sv = self._sum(i, (self.mV[j] * w)[:, None] * self.vt[j])
         return w, sv
 
@@ -175,7 +191,7 @@ class NumpySPH:
         B, gam = self.d["stiffness"], self.d["exponent"]
         self.p[fl] = np.maximum(B * ((self.rhot[fl] / self.rho0) ** gam - 1.0), 0.0)
         i, j, xij, r = self.pairs()
-        bd = is_bdy(self.t)
+        bd = is_bdy(self.t) | (self.t == RIGID)  # solver_sph_wc.py:90
         w, sv = self._bdy_common(i, j, xij, r)
         sp = self._sum(i, self.mV[j] * (self.p[j] + self.rhot[j] * self.g[1] * xij[:, 1]) * w)
         self.vt[bd] = 2 * self.v[bd] - sv[bd] * self.cspm[bd, None]
@@ -195,6 +211,15 @@ class NumpySPH:
         dv = self._sum(i, np.where(sel, vis + pc, 0.0)[:, None] * gw) + self._rep(i, j, xij, r)
         self.drho[fl] = dd[fl] * self.rhot[fl]
         self.dv[fl] = dv[fl] + self.g
+        # rigid: static d_vel = 0 (:125-126); dynamic g (:105-106) minus the
+        # fluid pair terms, scattered from the fluid side (:68-71)
+        rig = self.t == RIGID
+        self.dv[rig] = 0.0
+        if self.rig_dyn.any():
+            onto = sel & self.rig_dyn[j]
+            back = np.zeros((self.n, 3))
+            np.add.at(back, j[onto], ((vis + pc)[:, None] * gw)[onto])
+            self.dv[self.rig_dyn] = self.g - back[self.rig_dyn]
 
     def _rep(self, i, j, xij, r):
         """Repulsive force on each particle from its repulsive neighbours
@@ -400,6 +425,36 @@ class NumpySPH:
             self.lf_full()
         self.advect_pos()
         self.advect_something()
+        self.solve_rigid()
+
+    def solve_rigid(self):
+        """solve_constraints (solver_sph_base.py:479-500): cm, A, the polar
+        factor R = U V^T of A's SVD (a proper rotation), x <- cm + R q."""
+        for o, cm0 in self.rigid:
+            sel = (self.t == RIGID) & (self.obj == o)
+            m = self.mass[sel]
+            cm = (m[:, None] * self.x[sel]).sum(axis=0) / m.sum()
+            q = self.x0[sel] - cm0
+            p = self.x[sel] - cm
+            A = np.einsum("n,na,nb->ab", self.mV[sel] * self.rho[sel], p, q)
+            if self.dim == 2:
+                U, _, Vt = np.linalg.svd(A[:2, :2])
+                R2 = U @ Vt
+                if np.linalg.det(R2) < 0:
+                    U[:, 1] *= -1
+                    R2 = U @ Vt
+                R = np.eye(3)
+                R[:2, :2] = R2
+            else:
+                U, _, Vt = np.linalg.svd(A)
+                R = U @ Vt
+                if np.linalg.det(R) < 0:
+                    U[:, 2] *= -1
+                    R = U @ Vt
+            goal = cm + q @ R.T
+            self.x[sel] = self.x[sel] + (goal - self.x[sel])
+            if self.dim == 2:
+                self.x[sel, 2] = 0.0
 
     def stress6(self, which="sig"):
         s = getattr(self, which)

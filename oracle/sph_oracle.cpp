@@ -37,11 +37,12 @@ enum {
   F_POS = 0, F_VEL, F_DENSITY, F_MASS, F_M_V, F_PRESSURE, F_TYPE, F_ID0,
   F_OBJ_ID, F_STRESS, F_STRESS_TMP, F_D_STRESS, F_STRAIN_EQU, F_STRAIN_EQU_P,
   F_D_STRAIN_EQU, F_D_STRAIN_EQU_P, F_FLAG_RETMAP, F_CSPM_F, F_D_DENSITY,
-  F_D_VEL, F_DENSITY_TMP, F_V_TMP, F_CELL, F_OFFSET, F_CELL_START, F_CSPM_L
+  F_D_VEL, F_DENSITY_TMP, F_V_TMP, F_CELL, F_OFFSET, F_CELL_START, F_CSPM_L,
+  F_X0 = 28  // initial positions, indexed by id0 (generation order)
 };
 enum {
   P_SORT = 0, P_SHEPARD, P_INIT_TMP, P_ONE_STEP, P_LF_HALF, P_LF_FULL,
-  P_ADVECT_POS, P_ADVECT_SOMETHING, P_STEP, P_SE, P_ENFORCE_BOUNDARY
+  P_ADVECT_POS, P_ADVECT_SOMETHING, P_STEP, P_SE, P_ENFORCE_BOUNDARY, P_SOLVE_RIGID
 };
 constexpr int FLUID = 1, SOIL = 2, RIGID = 11, DUMMY = -1, REP = -2;
 constexpr int WC = 1, MUI = 2, DP = 3;
@@ -49,6 +50,50 @@ constexpr int WC = 1, MUI = 2, DP = 3;
 inline bool is_flow(int t) { return t == FLUID || t == SOIL; }
 inline bool is_real(int t) { return t > 0; }
 inline bool is_bdy(int t) { return t == DUMMY || t == REP; }
+
+// Orthogonal factor of the polar decomposition A = R S (ti.polar_decompose
+// in solve_constraints, solver_sph_base.py:490).  2D (the xy block, R_zz =
+// 1): the nearest rotation, (a + d, c - b) normalised.  3D: Newton's
+// iteration X <- (X + X^-T) / 2 (Higham 1986) from X = A.  A singular A
+// gives I.
+inline void polar_rotation(const double A[9], int dim, double R[9]) {
+  for (int c = 0; c < 9; ++c) R[c] = (c % 4 == 0) ? 1.0 : 0.0;
+  if (dim == 2) {
+    const double s = A[0] + A[4], t = A[3] - A[1];
+    const double nrm = std::sqrt(s * s + t * t);
+    if (nrm == 0.0) return;
+    R[0] = s / nrm;
+    R[1] = -t / nrm;
+    R[3] = t / nrm;
+    R[4] = s / nrm;
+    return;
+  }
+  double X[9];
+  for (int c = 0; c < 9; ++c) X[c] = A[c];
+  for (int it = 0; it < 100; ++it) {
+    double cf[9];  // cofactors: X^-T = cof(X) / det(X)
+    cf[0] = X[4] * X[8] - X[5] * X[7];
+    cf[1] = X[5] * X[6] - X[3] * X[8];
+    cf[2] = X[3] * X[7] - X[4] * X[6];
+    cf[3] = X[2] * X[7] - X[1] * X[8];
+    cf[4] = X[0] * X[8] - X[2] * X[6];
+    cf[5] = X[1] * X[6] - X[0] * X[7];
+    cf[6] = X[1] * X[5] - X[2] * X[4];
+    cf[7] = X[2] * X[3] - X[0] * X[5];
+    cf[8] = X[0] * X[4] - X[1] * X[3];
+    const double det = X[0] * cf[0] + X[1] * cf[1] + X[2] * cf[2];
+    if (det == 0.0) return;
+    double d = 0.0;
+    for (int c = 0; c < 9; ++c) {
+      const double xn = 0.5 * (X[c] + cf[c] / det);
+      const double e = std::fabs(xn - X[c]);
+      d = e > d ? e : d;
+      X[c] = xn;
+    }
+    if (d <= 1e-15) break;
+  }
+  for (int c = 0; c < 9; ++c) R[c] = X[c];
+}
 
 // Stress component order xx,yy,zz,xy,yz,zx <-> 3x3 index.
 inline int sidx(int a, int b) {
@@ -98,6 +143,10 @@ struct Sim {
   std::vector<double> dsigRK[6];
   std::vector<R> xsx, xsy, xsz;
   std::vector<R> Lc[9];  // CSPM_L (row-major 3x3), kernelCorrection 1
+  // dynamic rigid objects: ids, rest centres of mass; initial positions by id0
+  int nrig = 0, rig_obj[4] = {0, 0, 0, 0};
+  double rig_cm0[4][3] = {};
+  std::vector<double> x0[3];
   std::vector<int64_t> start;  // size C+1, exclusive scan of cell counts
 
   bool f32_state() const { return sizeof(R) == 4; }
@@ -151,6 +200,7 @@ struct Sim {
     for (int c = 0; c < 6; ++c) { zd(sig[c]); zd(sigt[c]); zd(dsig[c]); zd(dsigRK[c]); }
     zd(seq); zd(seqp); zd(dseq); zd(dseqp);
     for (int c = 0; c < 9; ++c) Lc[c].assign(n, (c % 4 == 0) ? R(1) : R(0));
+    for (int c = 0; c < 3; ++c) x0[c].assign(n, 0.0);
     start.assign(C + 1, 0);
   }
 
@@ -477,10 +527,100 @@ struct Sim {
       if (type[i] == FLUID) pr[i] = eos(rhot[i]);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; ++i)
-      if (is_bdy(type[i])) wc_bdy(i);
+      if (is_bdy(type[i]) || type[i] == RIGID) wc_bdy(i);  // :90-103
 #pragma omp parallel for schedule(dynamic, 256)
-    for (int64_t i = 0; i < n; ++i)
+    for (int64_t i = 0; i < n; ++i) {
       if (type[i] == FLUID) wc_fluid(i);
+      else if (type[i] == RIGID) wc_rigid(i);
+    }
+  }
+  // Rigid particle i (solver_sph_wc.py:105-106, 68-71, 125-126): static ->
+  // d_vel 0; dynamic -> g minus the calc_d_vel_task term of every fluid
+  // neighbour j, formed from j's side (x_ji = x_j - x_i, j's density_tmp in
+  // the wall viscosity).  The reference subtracts these in loop B with
+  // atomic updates in no fixed order; here i gathers them in its own
+  // neighbour order.
+  int add_rigid(int o, const double cm[3]) {  // calc_rigid_rest_m, :517-518
+    int k = 0;
+    while (k < nrig && rig_obj[k] != o) ++k;
+    if (k == 4) return -1;
+    rig_obj[k] = o;
+    for (int a = 0; a < 3; ++a) rig_cm0[k][a] = cm[a];
+    if (k == nrig) ++nrig;
+    return 0;
+  }
+  int rigid_slot(int64_t i) const {
+    if (type[i] != RIGID) return -1;
+    for (int k = 0; k < nrig; ++k)
+      if (rig_obj[k] == obj[i]) return k;
+    return -1;
+  }
+  void wc_rigid(int64_t i) {
+    drho[i] = 0;
+    if (rigid_slot(i) < 0) {
+      dvx[i] = dvy[i] = dvz[i] = R(0);
+      return;
+    }
+    R dv[3] = {grav[0], grav[1], grav[2]};
+    const R mVi = mV[i];
+    const R pri = pr[i] / (rhot[i] * rhot[i]);
+    walk(i, [&](int64_t j, R x, R y, R z, R r2) {
+      if (type[j] != FLUID) return;
+      const R xj = -x, yj = -y, zj = -z;
+      R rn = std::sqrt(r2);
+      R f = gradW_fac(rn);
+      R gx = f * xj, gy = f * yj, gz = f * zj;
+      R ux = vtx[j] - vtx[i], uy = vty[j] - vty[i], uz = vtz[j] - vtz[i];
+      R vx_ = ux * xj + uy * yj + uz * zj;
+      R mn = vx_ < R(0) ? vx_ : R(0);
+      R rr = rho0 / rhot[j];
+      R vis = ((visc_c1 * mVi) * rr) * mn / (r2 + visc_c2);
+      R pc = (-rho0 * mVi) * (pr[j] / (rhot[j] * rhot[j]) + pri);
+      R sc = vis + pc;
+      dv[0] -= sc * gx;
+      dv[1] -= sc * gy;
+      dv[2] -= sc * gz;
+    });
+    dvx[i] = dv[0];
+    dvy[i] = dv[1];
+    dvz[i] = dv[2];
+  }
+  // solve_rigid_body / solve_constraints (solver_sph_base.py:472-510), in
+  // FP64: cm = sum m x / sum m; A = sum m_V rho (x - cm) (x0 - cm0)^T;
+  // R = polar rotation of A; x += cm + R (x0 - cm0) - x.
+  void solve_rigid() {
+    for (int k = 0; k < nrig; ++k) {
+      double m = 0, mx[3] = {0, 0, 0};
+      for (int64_t i = 0; i < n; ++i) {
+        if (rigid_slot(i) != k) continue;
+        m += (double)mass[i];
+        for (int a = 0; a < 3; ++a) mx[a] += (double)mass[i] * abs_or0(a, i);
+      }
+      double cm[3] = {mx[0] / m, mx[1] / m, mx[2] / m}, A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int64_t i = 0; i < n; ++i) {
+        if (rigid_slot(i) != k) continue;
+        const double w = (double)mV[i] * (double)rho[i];
+        double q[3], p[3];
+        for (int a = 0; a < 3; ++a) {
+          q[a] = x0[a][id0[i]] - rig_cm0[k][a];
+          p[a] = abs_or0(a, i) - cm[a];
+        }
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) A[3 * a + b] += w * p[a] * q[b];
+      }
+      double Rm[9];
+      polar_rotation(A, dim, Rm);
+      for (int64_t i = 0; i < n; ++i) {
+        if (rigid_slot(i) != k) continue;
+        double q[3];
+        for (int a = 0; a < 3; ++a) q[a] = x0[a][id0[i]] - rig_cm0[k][a];
+        for (int a = 0; a < dim; ++a) {
+          const double goal = cm[a] + (Rm[3 * a] * q[0] + Rm[3 * a + 1] * q[1] + Rm[3 * a + 2] * q[2]);
+          const double x = abs_pos(a, i);
+          set_abs(a, i, x + (goal - x) * 1.0);
+        }
+      }
+    }
   }
 
   // ------------------------------------------------------------------ muI
@@ -929,17 +1069,20 @@ struct Sim {
     const R o = a == 0 ? ox[i] : a == 1 ? oy[i] : oz[i];
     return P.vdomain_start[a] + (double)c * grid + (double)o;
   }
+  double abs_or0(int a, int64_t i) const { return (a < 2 || dim == 3) ? abs_pos(a, i) : 0.0; }
   void set_abs(int a, int64_t i, double x) {
     int32_t& c = a == 0 ? cx[i] : a == 1 ? cy[i] : cz[i];
     R& o = a == 0 ? ox[i] : a == 1 ? oy[i] : oz[i];
     o = (R)(x - P.vdomain_start[a] - (double)c * grid);
   }
-  // enforce_boundary_2D/3D (solver_sph_base.py:540-601), boundary == 1.
+  // enforce_boundary_2D/3D (solver_sph_base.py:540-601) for the particles
+  // of judge_enforce_bdy (:532-538): flow particles with boundary == 1,
+  // dynamic rigid particles always.
   void enforce_boundary() {
-    if (P.boundary != 1) return;
+    if (P.boundary != 1 && nrig == 0) return;
     double dist = P.particle_radius - 1e-8;
     for (int64_t i = 0; i < n; ++i) {
-      if (!is_flow(type[i])) continue;
+      if (!((P.boundary == 1 && is_flow(type[i])) || rigid_slot(i) >= 0)) continue;
       double pos[3] = {abs_pos(0, i), abs_pos(1, i), dim == 3 ? abs_pos(2, i) : 0.0};
       double nrm[3] = {0, 0, 0};
       if (pos[0] > P.domain_end[0] - dist) { nrm[0] += 1; set_abs(0, i, P.domain_end[0] - dist); }
@@ -967,6 +1110,7 @@ struct Sim {
     substep();
     advect_pos();
     advect_something();
+    solve_rigid();
     enforce_boundary();
   }
 
@@ -998,6 +1142,10 @@ struct Sim {
         for (int c = 0; c < 6; ++c) t[c][i] = rnd_mui(d[6 * i + c]);
     };
     switch (f) {
+      case F_X0:
+        for (int64_t i = 0; i < n; ++i)
+          for (int c = 0; c < 3; ++c) x0[c][i] = d[3 * i + c];
+        return 0;
       case F_POS:
         for (int64_t i = 0; i < n; ++i) {
           int32_t* cc[3] = {&cx[i], &cy[i], &cz[i]};
@@ -1056,6 +1204,10 @@ struct Sim {
       for (int64_t i = 0; i < n; ++i) for (int c = 0; c < 6; ++c) d[6 * i + c] = t[c][i];
     };
     switch (f) {
+      case F_X0:
+        for (int64_t i = 0; i < n; ++i)
+          for (int c = 0; c < 3; ++c) d[3 * i + c] = x0[c][i];
+        return 0;
       case F_POS:
         for (int64_t i = 0; i < n; ++i)
           for (int a = 0; a < 3; ++a) d[3 * i + a] = (a == 2 && dim == 2) ? 0.0 : abs_pos(a, i);
@@ -1104,6 +1256,7 @@ struct Sim {
       case P_ADVECT_POS: advect_pos(); return 0;
       case P_ADVECT_SOMETHING: advect_something(); return 0;
       case P_ENFORCE_BOUNDARY: enforce_boundary(); return 0;
+      case P_SOLVE_RIGID: solve_rigid(); return 0;
       case P_STEP: step(); return 0;
     }
     return -1;
@@ -1155,6 +1308,8 @@ int orc_step(void* h, int nsteps) {
   for (int k = 0; k < nsteps; ++k) orc_pass(h, P_STEP);
   return 0;
 }
+int orc_set_rigid_object(void* h, int obj, const double cm[3]) { DISPATCH(h, S.add_rigid(obj, cm)); }
+int orc_polar(const double* A, int dim, double* R) { return polar_rotation(A, dim, R), 0; }
 int orc_globals(void* h, double out[4]) { DISPATCH(h, (S.globals(out), 0)); }
 int64_t orc_neighbors(void* h, int64_t i, int64_t* out, int64_t cap) { DISPATCH(h, S.neighbors(i, out, cap)); }
 int64_t orc_escaped(void* h) { DISPATCH(h, S.escaped); }

@@ -17,6 +17,20 @@ def scene_cfg(name):
     return SimConfiger(os.path.join(SCENES, name + ".json"))
 
 
+def add_rigid_blocks(cfg, blocks, density=500.0):
+    """Append a rigid material (matType 11) and rigid blocks to a scene
+    config: ``blocks`` = [(objectId, translation, size, isDynamic), ...]."""
+    c = cfg.config
+    mat_id = len(c["Materials"])
+    c["Materials"].append({"matId": mat_id, "matType": 11, "color": [120, 80, 40],
+                           "density0": density})
+    for oid, tr, size, dyn in blocks:
+        c["Blocks"].append({"objectId": oid, "translation": list(tr), "size": list(size),
+                            "velocity": [0.0, 0.0, 0.0], "materialId": mat_id,
+                            "isDynamic": bool(dyn)})
+    return cfg
+
+
 def tiny_cfg(method=1, dim=2, xsph=False, r=None, domain=(0.2, 0.2, 0.06), block=(0.08, 0.08, 0.04),
              boundary=2, ti=2, kernel=1, kcorr=0):
     """A small dam-break / column scene for brute-force checks."""

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle, load_scene
-from tests.common import TOL, parity_err, scene_cfg
+from tests.common import TOL, add_rigid_blocks, parity_err, scene_cfg
 from tisphi_amd import _lib as L
 from tisphi_amd.scene import build_scene
 
@@ -42,6 +42,10 @@ CASES = {
     "dp2d_cspm": ("column_collapse_dp", {"kernelCorrection": 1}, 30, 0.2),
     "wc2d_mls": ("dam_break_2d", {"kernelCorrection": 2}, 100, 0.6),
     "mui2d_mls": ("column_collapse_mui", {"kernelCorrection": 2}, 30, 0.2),
+    # a dynamic box beside the water column and a static obstacle
+    "wc2d_rigid": ("dam_break_2d", {"rigid": [(10, (1.205, 0.05, 0.0), (0.1, 0.1, 0.1), True),
+                                              (11, (2.0, 0.0, 0.0), (0.1, 0.2, 0.1), False)]},
+                   150, 0.6),
 }
 
 STATE = ["cell", "offset", "vel", "density", "mass", "m_V", "type", "id0", "obj_id", "pressure",
@@ -54,6 +58,9 @@ def make_case(key):
     name, mod, pre, H = CASES[key]
     cfg = scene_cfg(name)
     if mod:
+        mod = dict(mod)
+        if "rigid" in mod:
+            add_rigid_blocks(cfg, mod.pop("rigid"))
         if "domainEnd" in mod or "is2D" in mod:
             cfg.config["Blocks"][0]["size"] = [0.2, 0.1, 0.05]
         cfg.config["Configuration"].update(mod)
@@ -64,23 +71,36 @@ def make_case(key):
     return sc, orc, H
 
 
-def state_fields(method):
+def state_fields(method, sc=None):
     f = list(STATE)
     if method != 1:
         f += SOIL_STATE
     if method == 3:
         f += DP_STATE
+    if sc is not None and any(r["dynamic"] for r in sc.rigid):
+        f += ["x0"]  # initial positions (by id0) of shape matching
     return f
+
+
+def new_oracle(sc):
+    o = Oracle(sc.desc(), 32)
+    for r in sc.rigid:
+        if r["dynamic"]:
+            o.set_rigid_object(r["obj"], r["rest_cm"])
+    return o
 
 
 def new_engine(sc):
     eng = L.Engine(sc.method, sc.geo.dim, sc.n, sc.geo.grid_num_total, 1)
     slot = eng.add_scene(sc.desc())
+    for r in sc.rigid:
+        if r["dynamic"]:
+            eng.set_rigid_object(slot, r["obj"], r["rest_cm"])
     return eng, slot
 
 
-def copy_state(orc, eng, slot, method):
-    for f in state_fields(method):
+def copy_state(orc, eng, slot, method, sc=None):
+    for f in state_fields(method, sc):
         eng.upload(slot, f, orc.get(f))
 
 
@@ -109,7 +129,7 @@ def case(request):
 def pass_fields(method, pass_name):
     base = {"shepard": ["cspm_f"], "init_tmp": ["density_tmp", "v_tmp"],
             "lf_half": ["density_tmp", "v_tmp", "m_V"], "lf_full": ["density", "vel", "m_V"],
-            "advect_pos": ["offset"]}
+            "advect_pos": ["offset"], "solve_rigid": ["offset"], "enforce_boundary": ["offset", "vel"]}
     if pass_name == "one_step":
         f = ["pressure", "density_tmp", "v_tmp", "d_density", "d_vel"]
         if method != 1:
@@ -130,9 +150,9 @@ def pass_fields(method, pass_name):
 def test_sort_bit_exact_and_neighbor_sets(case):
     key, sc, orc, H = case
     eng, slot = new_engine(sc)
-    copy_state(orc, eng, slot, sc.method)
-    o2 = Oracle(sc.desc(), 32)
-    for f in state_fields(sc.method):
+    copy_state(orc, eng, slot, sc.method, sc)
+    o2 = new_oracle(sc)
+    for f in state_fields(sc.method, sc):
         o2.set(f, orc.get(f))
     o2.run("sort")
     eng.run("sort")
@@ -159,20 +179,27 @@ def test_single_pass_parity(case):
     key, sc, orc, H = case
     extra = {"shepard": ["cspm_L"]} if sc.kernel_corr != 0 else {}
     eng, slot = new_engine(sc)
-    copy_state(orc, eng, slot, sc.method)
-    o2 = Oracle(sc.desc(), 32)
-    for f in state_fields(sc.method):
+    copy_state(orc, eng, slot, sc.method, sc)
+    o2 = new_oracle(sc)
+    for f in state_fields(sc.method, sc):
         o2.set(f, orc.get(f))
-    for ps in ["sort", "shepard", "init_tmp", "one_step", "lf_half", "one_step", "lf_full",
-               "advect_pos", "advect_something"]:
+    seq = ["sort", "shepard", "init_tmp", "one_step", "lf_half", "one_step", "lf_full",
+           "advect_pos", "advect_something"]
+    if any(r["dynamic"] for r in sc.rigid):
+        seq += ["solve_rigid", "enforce_boundary"]
+    for ps in seq:
         o2.run(ps)
         eng.run(ps)
         check_fields(sc, o2, eng, slot, H, pass_fields(sc.method, ps) + extra.get(ps, []), ps)
     # everything after the full sequence
-    check_fields(sc, o2, eng, slot, H, [f for f in state_fields(sc.method)
+    check_fields(sc, o2, eng, slot, H, [f for f in state_fields(sc.method, sc)
                                         if f not in ("cell", "type", "id0", "obj_id", "flag_retmap")],
                  "end of step")
     np.testing.assert_array_equal(eng.download(slot, "id0"), o2.get("id0"))
+    if key == "wc2d_rigid":  # the water pushes the dynamic box: d_vel != g
+        t, obj = eng.download(slot, "type"), eng.download(slot, "obj_id")
+        dv = eng.download(slot, "d_vel")[(t == 11) & (obj == 10)]
+        assert np.max(np.abs(dv - np.array(sc.gravity))) > 0.1
     if sc.method == 3:
         np.testing.assert_array_equal(eng.download(slot, "flag_retmap"), o2.get("flag_retmap"))
     eng.close()
@@ -183,7 +210,7 @@ def test_fused_step_equals_pass_sequence_and_graph(case):
     engs = []
     for mode in ("passes", "eager", "graph"):
         eng, slot = new_engine(sc)
-        copy_state(orc, eng, slot, sc.method)
+        copy_state(orc, eng, slot, sc.method, sc)
         if mode == "passes":
             for _ in range(3):
                 eng.run("step")
@@ -194,15 +221,15 @@ def test_fused_step_equals_pass_sequence_and_graph(case):
         engs.append((eng, slot))
     ref_e, ref_s = engs[0]
     for eng, slot in engs[1:]:
-        for f in state_fields(sc.method):
+        for f in state_fields(sc.method, sc):
             np.testing.assert_array_equal(eng.download(slot, f), ref_e.download(ref_s, f), err_msg=f)
     # and the fused step tracks the oracle within tolerance
-    o2 = Oracle(sc.desc(), 32)
-    for f in state_fields(sc.method):
+    o2 = new_oracle(sc)
+    for f in state_fields(sc.method, sc):
         o2.set(f, orc.get(f))
     o2.step(1)
     eng, slot = new_engine(sc)
-    copy_state(orc, eng, slot, sc.method)
+    copy_state(orc, eng, slot, sc.method, sc)
     eng.step(1)
     eng.sync()
     check_fields(sc, o2, eng, slot, H, ["density", "pressure", "d_vel", "d_density", "vel"], "fused step")
@@ -240,10 +267,10 @@ def test_neighbor_list_path_equals_stencil_walk(case):
     for use_nl in (0, 1, 2):
         eng, slot = new_engine(sc)
         eng.set_neighbor_list(use_nl)
-        copy_state(orc, eng, slot, sc.method)
+        copy_state(orc, eng, slot, sc.method, sc)
         eng.step(3)
         eng.sync()
-        res.append({f: eng.download(slot, f) for f in state_fields(sc.method)})
+        res.append({f: eng.download(slot, f) for f in state_fields(sc.method, sc)})
         eng.close()
     for r in res[1:]:
         for f in res[0]:

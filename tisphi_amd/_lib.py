@@ -27,14 +27,15 @@ FIELDS = dict(pos=(0, "f8", 3), vel=(1, "f8", 3), density=(2, "f8", 1), mass=(3,
               cspm_L=(25, "f8", 9), val=(26, "f8", 1), color=(27, "f8", 3), x0=(28, "f8", 3),
               color0=(29, "f8", 3))
 PASSES = dict(sort=0, shepard=1, init_tmp=2, one_step=3, lf_half=4, lf_full=5,
-              advect_pos=6, advect_something=7, step=8, se=9, enforce_boundary=10)
+              advect_pos=6, advect_something=7, step=8, se=9, enforce_boundary=10, solve_rigid=11)
 
 EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_engine_create",
            "tsph_engine_destroy", "tsph_engine_bytes", "tsph_add_scene", "tsph_remove_scene",
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
            "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list", "tsph_globals",
            "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats", "tsph_create",
-           "tsph_destroy", "tsph_assign_value_color", "tsph_v_maxmin", "tsph_set_color"]
+           "tsph_destroy", "tsph_assign_value_color", "tsph_v_maxmin", "tsph_set_color",
+           "tsph_set_rigid_object"]
 
 
 class SceneDesc(ctypes.Structure):
@@ -121,6 +122,7 @@ def load():
         "tsph_assign_value_color": ([vp, i32, i32, i32, ctypes.c_uint32], i32),
         "tsph_v_maxmin": ([vp, i32, ctypes.c_double, ctypes.c_double, i32, i32, dp, dp], i32),
         "tsph_set_color": ([vp, i32, i32, i32], i32),
+        "tsph_set_rigid_object": ([vp, i32, i32, dp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -256,6 +258,11 @@ class Engine:
                                          s2.ctypes.data))
         return c, s1, s2
 
+    def set_rigid_object(self, slot, obj, rest_cm):
+        """Register a dynamic rigid object (isDynamic block) of ``slot``."""
+        cm = (ctypes.c_double * 3)(*[float(v) for v in rest_cm])
+        check(load().tsph_set_rigid_object(self.h, int(slot), int(obj), cm))
+
     # colour feed (solver.assign_value_color, ps.v_maxmin, ps.set_color)
     def assign_value_color(self, slot, color_title, show_bdy=False, seed=0):
         check(load().tsph_assign_value_color(self.h, int(slot), int(color_title),
@@ -290,3 +297,9 @@ def upload_scene(eng, slot, scene, id0=None):
     eng.upload(slot, "obj_id", scene.obj_id)
     if scene.method != 1:
         eng.upload(slot, "stress", scene.stress)
+    dyn = [r for r in getattr(scene, "rigid", []) if r["dynamic"]]
+    if dyn:
+        # shape matching reads the initial positions (by id0)
+        eng.upload(slot, "x0", scene.pos)
+        for r in dyn:
+            eng.set_rigid_object(slot, r["obj"], r["rest_cm"])

@@ -24,6 +24,10 @@ constexpr int kWave = 64;
 __device__ __forceinline__ bool is_flow(int t) { return t == kFluid || t == kSoil; }
 __device__ __forceinline__ bool is_real(int t) { return t > 0; }
 __device__ __forceinline__ bool is_bdy(int t) { return t == kDummy || t == kRep; }
+// Particles the WC pair passes treat as walls (dummy extrapolation, flow
+// neighbours only): boundary and rigid (solver_sph_wc.py:90).
+__device__ __forceinline__ bool is_solid(int t) { return is_bdy(t) || t == kRigid; }
+constexpr int kMaxRigid = 4;  // dynamic rigid objects per scene
 
 // Stress components xx,yy,zz,xy,yz,zx.
 __device__ __forceinline__ constexpr int sidx(int a, int b) {
@@ -48,7 +52,20 @@ struct SceneDev {
   float dt, half_dt, dt6, rho0, visc_c1, visc_c2, grav[3];
   float stiffness, exponent, vsound2, coh, mu, young, h2;
   double alpha, kc, G, K, sin_psi, eps_f;
+  // dynamic rigid objects (isDynamic blocks): object ids and rest centres
+  // of mass (calc_rigid_rest_m, solver_sph_base.py:467-470, 517-518)
+  int32_t nrig, rig_obj[kMaxRigid];
+  double rig_cm0[kMaxRigid][3];
 };
+
+// Index of particle p's object among the scene's dynamic rigid objects, -1
+// for a static rigid or any other particle.
+__device__ __forceinline__ int rigid_slot(const SceneDev& S, int type, int obj) {
+  if (type != kRigid) return -1;
+  for (int k = 0; k < S.nrig; ++k)
+    if (S.rig_obj[k] == obj) return k;
+  return -1;
+}
 
 // Particle state that survives a step (double-buffered; the counting sort
 // moves it from one buffer to the other).
@@ -105,6 +122,9 @@ struct PWork {
   double* x0;        // 3 x cap initial position, indexed by scene base + id0
   float4* c0;        // initial colour, indexed by scene base + id0
   double* crange;    // per slot: vmax, vmin of the last v_maxmin
+  // shape matching of dynamic rigid objects (allocated with the first one)
+  double* rigpart;   // per tile and object: 16 partial sums
+  double* rigout;    // per slot and object: centre of mass (3), rotation (9)
 };
 
 struct KArgs {

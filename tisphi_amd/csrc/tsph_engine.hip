@@ -42,7 +42,16 @@ int dalloc(T** p, size_t count, size_t& bytes) {
     *p = nullptr;
     return fail(TSPH_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
-  hipMemset(*p, 0, count * sizeof(T));
+  // The engine's stream does not synchronise with the null stream the
+  // clear runs on: wait for it here, or a later kernel or copy on the
+  // engine's stream could run before it and be overwritten by the zeros.
+  e = hipMemset(*p, 0, count * sizeof(T));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    hipFree(*p);
+    *p = nullptr;
+    return fail(TSPH_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  }
   bytes += count * sizeof(T);
   return TSPH_OK;
 }
@@ -86,9 +95,10 @@ struct Timing {
 // captured graph is valid only for the options it was captured with).
 struct StepFlags {
   int ti = 2;
-  bool xsph = false, collide = false, kcorr = false;
+  bool xsph = false, collide = false, kcorr = false, rigid = false;
   bool operator==(const StepFlags& o) const {
-    return ti == o.ti && xsph == o.xsph && collide == o.collide && kcorr == o.kcorr;
+    return ti == o.ti && xsph == o.xsph && collide == o.collide && kcorr == o.kcorr &&
+           rigid == o.rigid;
   }
 };
 
@@ -204,7 +214,7 @@ void free_work(PWork& w) {
   void* ptrs[] = {w.key, w.slot, w.tmp, w.sid, w.p, w.vp, w.Lc, w.cspm, w.drho, w.dv, w.sigt_f, w.sigt_d,
                   w.dsig_d, w.dseq, w.dseqp, w.xs, w.drho_rk, w.dv_rk, w.dsig_rk, w.count,
                   w.start, w.bsum, w.status, w.gpart, w.gout, w.nl, w.nlc, w.cflow,
-                  w.val, w.rgb, w.x0, w.c0, w.crange};
+                  w.val, w.rgb, w.x0, w.c0, w.crange, w.rigpart, w.rigout};
   for (void* p : ptrs)
     if (p) hipFree(p);
   w = PWork{};
@@ -231,6 +241,19 @@ int ensure_colour(tsph_engine* E) {
       (r = dalloc(&w.c0, n, b)) || (r = dalloc(&w.crange, 2 * (size_t)E->max_scenes, b)))
     return r;
   invalidate_graph(E);  // the kernel arguments captured in the graphs changed
+  return TSPH_OK;
+}
+
+// Shape-matching arrays, with the first dynamic rigid object (and the
+// initial positions they read, shared with the colour feed).
+int ensure_rigid(tsph_engine* E) {
+  int r = ensure_colour(E);
+  if (r || E->w.rigpart) return r;
+  size_t& b = E->bytes;
+  if ((r = dalloc(&E->w.rigpart, (size_t)E->tiles_alloc * kMaxRigid * kRigidSums, b)) ||
+      (r = dalloc(&E->w.rigout, (size_t)E->max_scenes * kMaxRigid * 12, b)))
+    return r;
+  invalidate_graph(E);
   return TSPH_OK;
 }
 
@@ -405,21 +428,31 @@ struct Seq {
     E->fresh_sort = false;
     L.go("enforce_boundary", k_enforce_boundary, tb(), E->elem_block, E->args());
   }
+  // solve_rigid_body (solver_sph_base.py:472-475): shape matching of the
+  // dynamic rigid objects.
+  void solve_rigid() {
+    E->fresh_sort = false;
+    KArgs a = E->args();
+    L.go("rigid", k_rigid_tile, E->nblocks_tiles(256), 256, a);
+    L.go("rigid", k_rigid_solve, E->max_scenes * kMaxRigid, 256, a, E->max_scenes);
+    L.go("rigid", k_rigid_apply, tb(), E->elem_block, a);
+  }
   // SPHBase.step (solver_sph_base.py:41-51) as the pass sequence, one
   // launch per reference kernel.
-  void step_passes(int ti, bool xsph, bool collide) {
+  void step_passes(int ti, bool xsph, bool collide, bool rigid) {
     sort();
     shepard();
     init_tmp();
     substep(ti);
     advect_pos(xsph);
     advect_something();
-    if (collide) enforce_boundary();
+    if (rigid) solve_rigid();
+    if (collide || rigid) enforce_boundary();
   }
   // The same step with elementwise passes fused into their neighbours
   // (bitwise identical results, fewer launches); all resident scenes share
   // the engine's time-integration scheme.
-  void step(int ti, bool xsph, bool collide) {
+  void step(int ti, bool xsph, bool collide, bool rigid) {
     sort(true);
     const bool shep = E->method == kWC;  // fluid Shepard sums in the fluid pass
     shepard(shep);
@@ -445,7 +478,8 @@ struct Seq {
       L.go("mui_regu", k_mui_regu<DIM>, pb(), E->pair_block, a);
       L.go("mui_adv", k_mui_adv2, tb(), E->elem_block, a);
     }
-    if (collide) enforce_boundary();
+    if (rigid) solve_rigid();
+    if (collide || rigid) enforce_boundary();
   }
   void globals() {
     KArgs a = E->args();
@@ -464,6 +498,7 @@ StepFlags step_flags(tsph_engine* E) {
     f.xsph = f.xsph || E->scenes[s].xsph;
     f.collide = f.collide || E->scenes[s].boundary == 1;
     f.kcorr = f.kcorr || E->scenes[s].kcorr != 0;
+    f.rigid = f.rigid || E->scenes[s].nrig > 0;
   }
   return f;
 }
@@ -471,8 +506,8 @@ StepFlags step_flags(tsph_engine* E) {
 int run_step_eager(tsph_engine* E) {
   Launcher L{E};
   StepFlags f = step_flags(E);
-  if (E->dim == 2) Seq<2>{E, L}.step(f.ti, f.xsph, f.collide);
-  else Seq<3>{E, L}.step(f.ti, f.xsph, f.collide);
+  if (E->dim == 2) Seq<2>{E, L}.step(f.ti, f.xsph, f.collide, f.rigid);
+  else Seq<3>{E, L}.step(f.ti, f.xsph, f.collide, f.rigid);
   return L.status;
 }
 
@@ -831,12 +866,13 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
       std::vector<int8_t> v;
       if (up) {
         // A scene holds the material of its method and boundary particles
-        // only (WC: fluid; muI / DP: soil); the pair kernels rely on it.
+        // only (WC: fluid, and rigid blocks; muI / DP: soil); the pair
+        // kernels rely on it.
         const int flow = E->method == kWC ? kFluid : kSoil;
         v.resize(n);
         for (int64_t i = 0; i < n; ++i) {
           const int t = hi[i];
-          if (t != flow && t != kDummy && t != kRep)
+          if (t != flow && t != kDummy && t != kRep && !(t == kRigid && E->method == kWC))
             return fail(TSPH_ERR_UNSUPPORTED, "particle type " + std::to_string(t) +
                                                   " in a scene of method " + std::to_string(E->method) +
                                                   " (only its own material and boundary particles)");
@@ -1138,7 +1174,8 @@ int tsph_run_pass(void* engine, int pass_id) {
       case TSPH_PASS_ADVECT_POS: seq.advect_pos(f.xsph); break;
       case TSPH_PASS_ADVECT_SOMETHING: seq.advect_something(); break;
       case TSPH_PASS_ENFORCE_BOUNDARY: seq.enforce_boundary(); break;
-      case TSPH_PASS_STEP: seq.step_passes(f.ti, f.xsph, f.collide); break;
+      case TSPH_PASS_STEP: seq.step_passes(f.ti, f.xsph, f.collide, f.rigid); break;
+      case TSPH_PASS_SOLVE_RIGID: seq.solve_rigid(); break;
       default: L.status = fail(TSPH_ERR_INVALID_ARG, "bad pass id");
     }
   };
@@ -1199,6 +1236,27 @@ int tsph_globals(void* engine, int slot, double out[4]) {
   if (L.status) return L.status;
   HIP_TRY(hipMemcpyAsync(out, E->w.gout + 4 * slot, 32, hipMemcpyDeviceToHost, E->stream));
   HIP_TRY(hipStreamSynchronize(E->stream));
+  return TSPH_OK;
+}
+
+int tsph_set_rigid_object(void* engine, int slot, int object_id, const double rest_cm[3]) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!slot_ok(E, slot) || !rest_cm) return fail(TSPH_ERR_INVALID_ARG, "bad argument");
+  if (E->method != kWC) return fail(TSPH_ERR_UNSUPPORTED, "rigid bodies are built for WCSPH scenes only");
+  SceneDev& S = E->scenes[slot];
+  int k = 0;
+  while (k < S.nrig && S.rig_obj[k] != object_id) ++k;  // a known object: new rest centre
+  if (k == kMaxRigid)
+    return fail(TSPH_ERR_CAPACITY, "at most " + std::to_string(kMaxRigid) + " dynamic rigid objects per scene");
+  int r = ensure_rigid(E);
+  if (r) return r;
+  S.rig_obj[k] = object_id;
+  for (int c = 0; c < 3; ++c) S.rig_cm0[k][c] = rest_cm[c];
+  if (k == S.nrig) S.nrig++;
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  HIP_TRY(hipMemcpy(E->d_scenes, E->scenes.data(), E->scenes.size() * sizeof(SceneDev),
+                    hipMemcpyHostToDevice));
+  invalidate_graph(E);
   return TSPH_OK;
 }
 

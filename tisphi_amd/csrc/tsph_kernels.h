@@ -399,7 +399,7 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
   const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
-  const bool bdy = is_bdy(type[p]);
+  const bool bdy = is_solid(type[p]);  // boundary or rigid: flow neighbours only
   const int cap = a.nl_cap;
   uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
   // A boundary particle only records flow neighbours: none can exist if
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
     decode_cell(S, c, cx, cy, cz);
     key = S.cbase + c;
     P = pos[p];
-    bdy = is_bdy(type[p]);
+    bdy = is_solid(type[p]);
   }
   if (act && bdy && use_cflow && !flow_in_reach<DIM>(S, a.w.cflow, cx, cy, cz)) {
     // as in k_nl_build: no flow cell in reach, no neighbours to record
@@ -698,7 +698,7 @@ template <int DIM, bool NL>
 __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (!is_bdy(a.cur.type[p])) return;
+  if (!is_solid(a.cur.type[p])) return;  // boundary and rigid (solver_sph_wc.py:90-103)
   const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
@@ -793,14 +793,62 @@ __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, 
   if (SHEP) a.w.cspm[p] = shep != 0.0f ? 1.0f / shep : 1.0f;
 }
 
+// d_vel of a dynamic rigid particle j: g (loop A, solver_sph_wc.py:105-106)
+// minus the pair force of every fluid neighbour i (loop B, :68-71, where
+// fluid i subtracts its calc_d_vel_task term from j).  The reference
+// scatters with atomic updates in no fixed order; here j gathers the same
+// terms, formed from i's side (x_ij = x_i - x_j, i's density_tmp in the
+// wall viscosity), in its own neighbour order.
+template <int DIM, bool NL, int KT>
+__device__ __forceinline__ float4 wc_rigid_force(const KArgs& a, const SceneDev& S, int p) {
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float4* __restrict__ vp = a.w.vp;
+  const float4 vj = vtmp[p];
+  const float mVj = C.P.w;
+  const float prj = fabsf(vp[p].w);  // p_j / rho_j^2 of the wall record
+  const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
+  float dvx = S.grav[0], dvy = S.grav[1], dvz = S.grav[2];
+  neighbours<DIM, NL>(S, a, p, C, [&](int i, float x, float y, float z, float r2, float, int) {
+    if (type[i] != kFluid) return;
+    const float xi = -x, yi = -y, zi = -z;  // x_i - x_j
+    const float rn = sqrtf(r2);
+    const float f = kernel_gradfac_t<KT>(S, rn);
+    const float gx = f * xi, gy = f * yi, gz = f * zi;
+    const float4 vi = vtmp[i];
+    const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
+    const float vx_ = ux * xi + uy * yi + uz * zi;
+    const float mn = vx_ < 0.0f ? vx_ : 0.0f;
+    const float rr = rho0 / vi.w;
+    const float vis = ((c1 * mVj) * rr) * mn / (r2 + c2);
+    const float pc = (-rho0 * mVj) * (vp[i].w + prj);
+    const float sc = vis + pc;
+    dvx -= sc * gx;
+    dvy -= sc * gy;
+    dvz -= sc * gz;
+  });
+  return make_float4(dvx, dvy, dvz, 0.0f);
+}
+
 template <int DIM, bool NL, bool SHEP>
 __global__ __launch_bounds__(256) void k_wc_fluid(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (a.cur.type[p] != kFluid) {
-    // Rates of other particles are never written by the reference: zero.
+  const int tp = a.cur.type[p];
+  if (tp != kFluid) {
+    // Rates of other particles are never written by the reference: zero;
+    // a static rigid particle's d_vel is zero too (solver_sph_wc.py:125-126).
+    float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tp == kRigid) {
+      const SceneDev S = a.scenes[s];
+      if (rigid_slot(S, tp, a.cur.obj[p]) >= 0) {
+        if (S.kernel == 1) dv = wc_rigid_force<DIM, NL, 1>(a, S, p);
+        else dv = wc_rigid_force<DIM, NL, 0>(a, S, p);
+      }
+    }
     a.w.drho[p] = 0.0f;
-    a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.w.dv[p] = dv;
     return;
   }
   const SceneDev S = a.scenes[s];
@@ -1465,7 +1513,11 @@ __global__ __launch_bounds__(256) void k_enforce_boundary(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   const SceneDev S = a.scenes[s];
-  if (S.boundary != 1 || !is_flow(a.cur.type[p])) return;
+  // judge_enforce_bdy (solver_sph_base.py:532-538): flow particles with the
+  // collision boundary, dynamic rigid particles always
+  const int t = a.cur.type[p];
+  const bool dyn = rigid_slot(S, t, a.cur.obj[p]) >= 0;
+  if (!((S.boundary == 1 && is_flow(t)) || dyn)) return;
   int cc[3];
   decode_cell(S, a.cur.cell[p], cc[0], cc[1], cc[2]);
   float4 P = a.cur.pos[p];
@@ -1690,6 +1742,162 @@ __global__ __launch_bounds__(256) void k_globals_scene(KArgs a, int nslots) {
     o[2] = ke;
     o[3] = m;
   }
+}
+
+// ============================================================== rigid bodies
+// solve_rigid_body / solve_constraints (solver_sph_base.py:472-510): shape
+// matching of every dynamic rigid object after advect_something, in FP64
+// (the reference's default_fp): centre of mass cm = sum m x / sum m,
+// A = sum m_V rho (x - cm) (x0 - cm0)^T, R the rotation of A's polar
+// decomposition, then x += (cm + R (x0 - cm0) - x).  One pass of per-tile
+// partial sums (sum m, m x, w x q^T, w q with w = m_V rho, q = x0 - cm0;
+// A = sum w x q^T - cm (sum w q)^T), one block per object in a fixed order,
+// then the per-particle move.
+__device__ __forceinline__ void abs_position(const SceneDev& S, int32_t cell, const float4& P,
+                                             double x[3]) {
+  int cc[3];
+  decode_cell(S, cell, cc[0], cc[1], cc[2]);
+  x[0] = S.vds[0] + (double)cc[0] * S.grid + (double)P.x;
+  x[1] = S.vds[1] + (double)cc[1] * S.grid + (double)P.y;
+  x[2] = S.dim == 3 ? S.vds[2] + (double)cc[2] * S.grid + (double)P.z : 0.0;
+}
+constexpr int kRigidSums = 16;
+__global__ __launch_bounds__(256) void k_rigid_tile(KArgs a) {
+  const int w = wave_id();
+  const int2 t = a.tiles[w];
+  if (t.x < 0) return;
+  const SceneDev S = a.scenes[t.x];
+  if (S.nrig == 0) return;  // wave-uniform
+  const int lane = (int)(threadIdx.x & 63);
+  const int p = t.y + lane;
+  int k = -1;
+  double v[kRigidSums];
+  for (int c = 0; c < kRigidSums; ++c) v[c] = 0.0;
+  if (p < S.pbase + S.n) k = rigid_slot(S, a.cur.type[p], a.cur.obj[p]);
+  if (k >= 0) {
+    double x[3];
+    const float4 P = a.cur.pos[p];
+    abs_position(S, a.cur.cell[p], P, x);
+    const double m = (double)a.cur.mass[p];
+    const double wv = (double)P.w * (double)a.cur.vel[p].w;
+    const size_t id = (size_t)S.pbase + (size_t)a.cur.id0[p];
+    const int st = a.stride;
+    double q[3];
+    for (int c = 0; c < 3; ++c) q[c] = a.w.x0[(size_t)c * st + id] - S.rig_cm0[k][c];
+    v[0] = m;
+    for (int c = 0; c < 3; ++c) v[1 + c] = m * x[c];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) v[4 + 3 * r + c] = wv * x[r] * q[c];
+    for (int c = 0; c < 3; ++c) v[13 + c] = wv * q[c];
+  }
+  for (int o = 0; o < S.nrig; ++o) {
+    double* out = a.w.rigpart + ((size_t)w * kMaxRigid + o) * kRigidSums;
+    const bool any = __any(k == o);
+    for (int c = 0; c < kRigidSums; ++c) {
+      double u = (k == o) ? v[c] : 0.0;
+      if (any) u = wave_sum(u);
+      if (lane == 0) out[c] = any ? u : 0.0;
+    }
+  }
+}
+
+// Orthogonal factor of the polar decomposition A = R S.  2D (the xy block;
+// R_zz = 1): the nearest rotation, (a + d, c - b) normalised.  3D: Newton's
+// iteration X <- (X + X^-T) / 2 (Higham 1986) from X = A.  A singular A
+// gives I.
+__device__ __forceinline__ void polar_rotation(const double A[9], int dim, double R[9]) {
+  for (int c = 0; c < 9; ++c) R[c] = (c % 4 == 0) ? 1.0 : 0.0;
+  if (dim == 2) {
+    const double s = A[0] + A[4], t = A[3] - A[1];
+    const double nrm = sqrt(s * s + t * t);
+    if (nrm == 0.0) return;
+    R[0] = s / nrm;
+    R[1] = -t / nrm;
+    R[3] = t / nrm;
+    R[4] = s / nrm;
+    return;
+  }
+  double X[9];
+  for (int c = 0; c < 9; ++c) X[c] = A[c];
+  for (int it = 0; it < 100; ++it) {
+    // cofactors: X^-T = cof(X) / det(X)
+    double cf[9];
+    cf[0] = X[4] * X[8] - X[5] * X[7];
+    cf[1] = X[5] * X[6] - X[3] * X[8];
+    cf[2] = X[3] * X[7] - X[4] * X[6];
+    cf[3] = X[2] * X[7] - X[1] * X[8];
+    cf[4] = X[0] * X[8] - X[2] * X[6];
+    cf[5] = X[1] * X[6] - X[0] * X[7];
+    cf[6] = X[1] * X[5] - X[2] * X[4];
+    cf[7] = X[2] * X[3] - X[0] * X[5];
+    cf[8] = X[0] * X[4] - X[1] * X[3];
+    const double det = X[0] * cf[0] + X[1] * cf[1] + X[2] * cf[2];
+    if (det == 0.0) return;
+    double d = 0.0;
+    for (int c = 0; c < 9; ++c) {
+      const double xn = 0.5 * (X[c] + cf[c] / det);
+      const double e = fabs(xn - X[c]);
+      d = e > d ? e : d;
+      X[c] = xn;
+    }
+    if (d <= 1e-15) break;
+  }
+  for (int c = 0; c < 9; ++c) R[c] = X[c];
+}
+
+__global__ __launch_bounds__(256) void k_rigid_solve(KArgs a, int nslots) {
+  const int s = blockIdx.x / kMaxRigid, o = blockIdx.x % kMaxRigid;
+  if (s >= nslots) return;
+  const SceneDev S = a.scenes[s];
+  if (!S.active || o >= S.nrig) return;
+  double v[kRigidSums];
+  for (int c = 0; c < kRigidSums; ++c) v[c] = 0.0;
+  for (int k = threadIdx.x; k < S.n_tiles; k += blockDim.x) {
+    const double* in = a.w.rigpart + ((size_t)(S.tile_first + k) * kMaxRigid + o) * kRigidSums;
+    for (int c = 0; c < kRigidSums; ++c) v[c] += in[c];
+  }
+  __shared__ double red[kRigidSums][256 / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int c = 0; c < kRigidSums; ++c) {
+    v[c] = wave_sum(v[c]);
+    if (lane == 0) red[c][wid] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < kRigidSums; ++c)
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) v[c] += red[c][k];
+  double cm[3], A[9], R[9];
+  for (int c = 0; c < 3; ++c) cm[c] = v[1 + c] / v[0];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) A[3 * r + c] = v[4 + 3 * r + c] - cm[r] * v[13 + c];
+  polar_rotation(A, S.dim, R);
+  double* out = a.w.rigout + ((size_t)s * kMaxRigid + o) * 12;
+  for (int c = 0; c < 3; ++c) out[c] = cm[c];
+  for (int c = 0; c < 9; ++c) out[3 + c] = R[c];
+}
+
+__global__ __launch_bounds__(256) void k_rigid_apply(KArgs a) {
+  int s, p;
+  if (!tile_thread(a, s, p)) return;
+  const SceneDev S = a.scenes[s];
+  const int k = rigid_slot(S, a.cur.type[p], a.cur.obj[p]);
+  if (k < 0) return;
+  const double* cr = a.w.rigout + ((size_t)s * kMaxRigid + k) * 12;
+  const size_t id = (size_t)S.pbase + (size_t)a.cur.id0[p];
+  const int st = a.stride;
+  double q[3], x[3];
+  for (int c = 0; c < 3; ++c) q[c] = a.w.x0[(size_t)c * st + id] - S.rig_cm0[k][c];
+  float4 P = a.cur.pos[p];
+  abs_position(S, a.cur.cell[p], P, x);
+  int cc[3];
+  decode_cell(S, a.cur.cell[p], cc[0], cc[1], cc[2]);
+  float* o[3] = {&P.x, &P.y, &P.z};
+  for (int r = 0; r < S.dim; ++r) {
+    const double goal = cr[r] + (cr[3 + 3 * r] * q[0] + cr[4 + 3 * r] * q[1] + cr[5 + 3 * r] * q[2]);
+    const double xn = x[r] + (goal - x[r]) * 1.0;
+    *o[r] = (float)(xn - S.vds[r] - (double)cc[r] * S.grid);
+  }
+  a.cur.pos[p] = P;
 }
 
 // ============================================================== colour feed

@@ -127,7 +127,8 @@ class ParticleSystem:
         self.blocks = config.get_blocks()
         self.bodies = config.get_bodies()
         self.object_collection = {}
-        self.object_id_rigid = set()
+        self.object_id_rigid = {r["obj"] for r in sc.rigid}
+        self.rigid_rest_cm = {r["obj"]: r["rest_cm"] for r in sc.rigid if r["dynamic"]}
         for b, n in zip(self.blocks, sc.block_counts):
             b["particleNum"] = n
             self.object_collection[b["objectId"]] = b

@@ -100,7 +100,13 @@ class SPHBase:
         self._pass("enforce_boundary")
 
     def solve_rigid_body(self):
-        """No rigid bodies are resident (rigid blocks are rejected at load)."""
+        """Shape matching of the dynamic rigid blocks (solver_sph_base.py:
+        472-510), one device pass (also part of every fused step)."""
+        self._pass("solve_rigid")
+
+    def init_rigid_body(self):
+        """The rest centres of mass (solver_sph_base.py:467-470) are
+        registered with the engine when the scene is uploaded."""
 
     # ----------------------------------------------------------- visuals
     def assign_value_color(self):

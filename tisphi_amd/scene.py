@@ -116,6 +116,9 @@ class Scene:
     obj_id: np.ndarray = None
     mat_id: np.ndarray = None
     color0: np.ndarray = None  # initial colour per particle (float32 values)
+    # rigid blocks: dicts obj, dynamic, rest_cm (mass-weighted mean of the
+    # initial positions, calc_rigid_rest_m, solver_sph_base.py:517-518)
+    rigid: list = field(default_factory=list)
     stress: np.ndarray = None  # (n, 6) xx,yy,zz,xy,yz,zx
 
     @property
@@ -176,9 +179,10 @@ def build_scene(cfg, generate=True) -> Scene:
         raise NotImplementedError("OBJ bodies are not built")
 
     mat_index, fluids, soils, rigids = set_material(cfg.get_materials())
-    if rigids and any(get_material(mat_index, (fluids, soils, rigids), b["materialId"])["matType"] == MAT_RIGID
-                      for b in cfg.get_blocks()):
-        raise NotImplementedError("rigid blocks are not built")
+    if method != METHOD_WC and rigids and any(
+            get_material(mat_index, (fluids, soils, rigids), b["materialId"])["matType"] == MAT_RIGID
+            for b in cfg.get_blocks()):
+        raise NotImplementedError("rigid blocks are built for WCSPH scenes only")
     if method == METHOD_WC:
         material = fluids[0]
         vsound = 60.0
@@ -263,6 +267,15 @@ def build_scene(cfg, generate=True) -> Scene:
     sc.color0 = np.concatenate(col) if col else np.zeros((0, 3), np.float32)
     sc.m_V = np.full(sc.n, geo.m_V0)
     sc.mass = geo.m_V0 * sc.density
+    for b in blocks:
+        mat = get_material(mat_index, lists, b["materialId"])
+        if mat["matType"] != MAT_RIGID:
+            continue
+        sel = (sc.mat_type == MAT_RIGID) & (sc.obj_id == b["objectId"])
+        m = sc.mass[sel]
+        cm = (m[:, None] * sc.pos[sel]).sum(axis=0) / m.sum()
+        sc.rigid.append(dict(obj=int(b["objectId"]), dynamic=bool(b.get("isDynamic", False)),
+                             rest_cm=cm))
     sc.stress = np.zeros((sc.n, 6))
     if method == METHOD_DP:
         sc.stress = init_stress_dp(sc)
