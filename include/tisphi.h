@@ -233,18 +233,19 @@ int tsph_set_neighbor_list(void* engine, int enable);
 int tsph_globals(void* engine, int slot, double out[4]);
 
 /* ---- rigid bodies ---------------------------------------------------------
- * Rigid blocks of a WCSPH scene (TSPH_MAT_RIGID particles; ParticleSystem
- * init_block, particle_system.py:150-174) are walls to the fluid passes
- * (dummy extrapolation, solver_sph_wc.py:90-103).  A static block keeps
- * d_vel = 0 (:125-126).  A dynamic block (isDynamic) is registered here
- * with its object id and its rest centre of mass (calc_rigid_rest_m,
+ * Rigid blocks (TSPH_MAT_RIGID particles; ParticleSystem init_block,
+ * particle_system.py:150-174) are walls to the flow passes (dummy
+ * extrapolation: solver_sph_wc.py:90-103, solver_sph_muI.py:96-110,
+ * solver_sph_dp.py:220-231).  A static block keeps d_vel = 0 (WC :125-126,
+ * muI :130-132, DP :272-274).  A dynamic block (isDynamic) is registered
+ * here with its object id and its rest centre of mass (calc_rigid_rest_m,
  * solver_sph_base.py:467-470, 517-518, the mass-weighted mean of the
- * initial positions): its particles take g minus the fluid's pair forces
- * (:105-106, 68-71), move with the integrator, are put back into shape by
- * solve_rigid_body (solver_sph_base.py:472-510; initial positions from
- * TSPH_F_X0) and are always kept inside the domain (judge_enforce_bdy,
- * :532-538).  At most 4 dynamic objects per scene; TSPH_ERR_UNSUPPORTED for
- * muI / DP engines. */
+ * initial positions): its particles take g minus the flow particles' pair
+ * forces (WC :68-71, 105-106; muI :45-46, 111-112; DP :164-165, 233-234),
+ * move with the integrator, are put back into shape by solve_rigid_body
+ * (solver_sph_base.py:472-510; initial positions from TSPH_F_X0) and are
+ * always kept inside the domain (judge_enforce_bdy, :532-538).  At most 4
+ * dynamic objects per scene. */
 int tsph_set_rigid_object(void* engine, int slot, int object_id, const double rest_cm[3]);
 /* (registering a known object id again replaces its rest centre) */
 

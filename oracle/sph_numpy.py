@@ -243,7 +243,7 @@ class NumpySPH:
         return L, dd, gw
 
     def _soil_bdy(self, i, j, xij, r, extrapolate_density):
-        bd = is_bdy(self.t)
+        bd = is_bdy(self.t) | (self.t == RIGID)  # solver_sph_muI.py:96, dp.py:221
         w, sv = self._bdy_common(i, j, xij, r)
         sr = self._sum(i, self.mV[j] * self.rhot[j] * w)
         gx = (self.rhot[j][:, None] * self.g[None, :]) * xij
@@ -266,6 +266,14 @@ class NumpySPH:
         dv = self._sum(i, np.where(sel[:, None], tmp, 0.0)) + self._rep(i, j, xij, r)
         damp = -5e-5 * np.sqrt(self.d["young"] / (self.rhot * self.h ** 2))
         self.dv[soil] = dv[soil] + self.g + damp[soil, None] * self.vt[soil]
+        # rigid: static 0; dynamic g minus the soil terms scattered onto it
+        # (solver_sph_muI.py:45-46, solver_sph_dp.py:164-165)
+        self.dv[self.t == RIGID] = 0.0
+        if self.rig_dyn.any():
+            onto = sel & self.rig_dyn[j]
+            back = np.zeros((self.n, 3))
+            np.add.at(back, j[onto], tmp[onto])
+            self.dv[self.rig_dyn] = self.g - back[self.rig_dyn]
 
     def mui_one_step(self):
         soil = self.t == SOIL

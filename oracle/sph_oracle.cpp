@@ -730,7 +730,7 @@ struct Sim {
       if (type[i] == SOIL) mui_soil1(i);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; ++i)
-      if (is_bdy(type[i])) soil_bdy(i, true);
+      if (is_bdy(type[i]) || type[i] == RIGID) soil_bdy(i, true);  // :96-110
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; ++i)
       if (type[i] == SOIL) {
@@ -738,7 +738,41 @@ struct Sim {
         soil_dvel_sum(i, dv);
         if (rep) add_rep_force(i, dv);
         soil_dvel_finish(i, dv);
+      } else if (type[i] == RIGID) {
+        soil_rigid(i);
       }
+  }
+  // Rigid particle i of a soil scene (solver_sph_muI.py:38-46, 111-112,
+  // 130-132; solver_sph_dp.py:158-165, 233-234, 272-274): static -> d_vel
+  // 0; dynamic -> g minus the calc_d_vel_from_stress_task term of every soil
+  // neighbour j, formed from j's side (x_ji, j's corrected gradient), in
+  // i's neighbour order (the reference subtracts them with atomics).
+  void soil_rigid(int64_t i) {
+    if (rigid_slot(i) < 0) {
+      dvx[i] = dvy[i] = dvz[i] = R(0);
+      return;
+    }
+    R dv[3] = {grav[0], grav[1], grav[2]};
+    const R ri2 = rhot[i] * rhot[i];
+    const R s = mV[i] * rhot[i];
+    walk(i, [&](int64_t j, R x, R y, R z, R r2) {
+      if (type[j] != SOIL) return;
+      R f = gradW_fac(std::sqrt(r2));
+      R gw[3] = {f * -x, f * -y, f * -z};
+      corr(j, gw);
+      R rj2 = rhot[j] * rhot[j];
+      for (int a = 0; a < 3; ++a) {
+        R acc = 0;
+        for (int b = 0; b < 3; ++b) {
+          int c = sidx(a, b);
+          acc += (s * ((R)sigt[c][i] / ri2 + (R)sigt[c][j] / rj2)) * gw[b];
+        }
+        dv[a] -= acc;
+      }
+    });
+    dvx[i] = dv[0];
+    dvy[i] = dv[1];
+    dvz[i] = dv[2];
   }
   // advect_something_func (muI :267-289) with D2.
   void mui_advect_something() {
@@ -899,10 +933,12 @@ struct Sim {
       }
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; ++i)
-      if (is_bdy(type[i])) soil_bdy(i, false);
+      if (is_bdy(type[i]) || type[i] == RIGID) soil_bdy(i, false);  // :220-231
 #pragma omp parallel for schedule(dynamic, 256)
-    for (int64_t i = 0; i < n; ++i)
+    for (int64_t i = 0; i < n; ++i) {
       if (type[i] == SOIL) dp_soil(i);
+      else if (type[i] == RIGID) soil_rigid(i);
+    }
   }
   void dp_advect_something() {  // solver_sph_dp.py:276-296
 #pragma omp parallel for

@@ -667,3 +667,30 @@ def test_rigid_oracle_matches_numpy_restatement(dim):
                             ("density", o.get("density"), ref.rho[idx], 1e-9)):
         s = np.max(np.abs(b))
         assert np.max(np.abs(a - b)) <= tol * s, (name, np.max(np.abs(a - b)), s)
+
+
+@pytest.mark.parametrize("method", [2, 3])
+def test_rigid_soil_oracle_matches_numpy_restatement(method):
+    """A dynamic block against a soil column (muI, DP) and a static one: the
+    oracle's gathered stress forces against the numpy restatement's
+    scattered ones (solver_sph_muI.py:38-46, solver_sph_dp.py:158-165)."""
+    from tests.common import add_rigid_blocks
+    blocks = [(5, (0.085, 0.02, 0.0), (0.03, 0.03, 0.03), True),
+              (6, (0.16, 0.0, 0.0), (0.02, 0.06, 0.02), False)]
+    sc = build_scene(add_rigid_blocks(tiny_cfg(method=method, dim=2, block=(0.08, 0.08, 0.04)), blocks))
+    d = sc.desc()
+    o = O.Oracle(d, 64)
+    O.load_scene(o, sc)
+    ref = NumpySPH(d, sc.pos, sc.vel, sc.density, sc.mass, sc.m_V, sc.mat_type, sc.stress,
+                   obj=sc.obj_id, rigid=sc.rigid)
+    for _ in range(25):
+        o.step(1)
+        ref.step()
+    idx = o.get("id0")
+    assert np.abs(ref.dv[ref.rig_dyn] - ref.g).max() > 1e-3  # the soil pushes
+    for name, a, b, tol in (("pos", o.get("pos"), ref.x[idx], 1e-9),
+                            ("vel", o.get("vel"), ref.v[idx], 1e-9),
+                            ("d_vel", o.get("d_vel"), ref.dv[idx], 1e-8),
+                            ("stress", o.get("stress"), ref.stress6("sig")[idx], 1e-8)):
+        s = np.max(np.abs(b))
+        assert np.max(np.abs(a - b)) <= tol * s, (name, np.max(np.abs(a - b)), s)

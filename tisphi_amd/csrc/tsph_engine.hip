@@ -865,14 +865,14 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
     case TSPH_F_TYPE: {
       std::vector<int8_t> v;
       if (up) {
-        // A scene holds the material of its method and boundary particles
-        // only (WC: fluid, and rigid blocks; muI / DP: soil); the pair
+        // A scene holds the material of its method, rigid blocks and
+        // boundary particles only (WC: fluid; muI / DP: soil); the pair
         // kernels rely on it.
         const int flow = E->method == kWC ? kFluid : kSoil;
         v.resize(n);
         for (int64_t i = 0; i < n; ++i) {
           const int t = hi[i];
-          if (t != flow && t != kDummy && t != kRep && !(t == kRigid && E->method == kWC))
+          if (t != flow && t != kDummy && t != kRep && t != kRigid)
             return fail(TSPH_ERR_UNSUPPORTED, "particle type " + std::to_string(t) +
                                                   " in a scene of method " + std::to_string(E->method) +
                                                   " (only its own material and boundary particles)");
@@ -1242,7 +1242,6 @@ int tsph_globals(void* engine, int slot, double out[4]) {
 int tsph_set_rigid_object(void* engine, int slot, int object_id, const double rest_cm[3]) {
   tsph_engine* E = (tsph_engine*)engine;
   if (!slot_ok(E, slot) || !rest_cm) return fail(TSPH_ERR_INVALID_ARG, "bad argument");
-  if (E->method != kWC) return fail(TSPH_ERR_UNSUPPORTED, "rigid bodies are built for WCSPH scenes only");
   SceneDev& S = E->scenes[slot];
   int k = 0;
   while (k < S.nrig && S.rig_obj[k] != object_id) ++k;  // a known object: new rest centre

@@ -399,7 +399,10 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
   const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
-  const bool bdy = is_solid(type[p]);  // boundary or rigid: flow neighbours only
+  // Boundary particles record flow neighbours only; rigid ones flow and
+  // rigid neighbours (the XSPH sum of a rigid particle runs over rigid
+  // ones); flow particles record all.
+  const bool bdy = is_bdy(type[p]), rig = type[p] == kRigid;
   const int cap = a.nl_cap;
   uint32_t* __restrict__ li = a.w.nl + nl_base(p, cap);
   // A boundary particle only records flow neighbours: none can exist if
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
   }
   ListAppender ap{li, cap};
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
-    if (bdy && !is_flow(type[j])) return;
+    if ((bdy && !is_flow(type[j])) || (rig && !(is_flow(type[j]) || type[j] == kRigid))) return;
     ap.add(nl_entry(j, code));
   };
   if (cull)
@@ -469,13 +472,14 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
   bool act = p < S.pbase + S.n;
   int cx = 0, cy = 0, cz = 0, key = 0;
   float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
-  bool bdy = false;
+  bool bdy = false, rig = false;
   if (act) {
     const int32_t c = a.cur.cell[p];
     decode_cell(S, c, cx, cy, cz);
     key = S.cbase + c;
     P = pos[p];
-    bdy = is_solid(type[p]);
+    bdy = is_bdy(type[p]);
+    rig = type[p] == kRigid;
   }
   if (act && bdy && use_cflow && !flow_in_reach<DIM>(S, a.w.cflow, cx, cy, cz)) {
     // as in k_nl_build: no flow cell in reach, no neighbours to record
@@ -485,9 +489,10 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
   uint32_t* __restrict__ li = a.w.nl + nl_base(act ? p : t.y, cap);
   ListAppender ap{li, cap};
   auto rec = [&](int j, float x, float y, float z, float r2, float mVj, int code) {
-    if (bdy && !is_flow(type[j])) return;
+    if ((bdy && !is_flow(type[j])) || (rig && !(is_flow(type[j]) || type[j] == kRigid))) return;
     ap.add(nl_entry(j, code));
   };
+  const float wmin = bdy ? 2.0f : (rig ? 1.0f : 0.0f);  // staged-record filter
   const int row = DIM == 3 ? cx * gy + cy : cx;
   const int rmin = wave_min(act ? row : INT32_MAX), rmax = wave_max(act ? row : INT32_MIN);
   if (rmin > rmax) return;  // no lane left to walk (wave-uniform)
@@ -526,11 +531,14 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
       const bool staged = cnt <= kStage;  // wave-uniform
       if (staged) {
         __syncthreads();  // the previous row offset's reads are done
-        // w = 1 for a flow particle (the only neighbours a boundary lane
-        // records), so the filter needs no global load per hit
+        // w = 2 for a flow particle (the only neighbours a boundary lane
+        // records), 1 for a rigid one (a rigid lane records both), 0
+        // otherwise: the lane's filter is one compare, w >= wmin, and needs
+        // no global load per hit
         for (int q = lane; q < cnt; q += 64) {
           float4 v = pos[jlo + q];
-          v.w = is_flow(type[jlo + q]) ? 1.0f : 0.0f;
+          const int tq = type[jlo + q];
+          v.w = is_flow(tq) ? 2.0f : (tq == kRigid ? 1.0f : 0.0f);
           sp[q] = v;
         }
         __syncthreads();
@@ -571,7 +579,7 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
             const float y = by - pj.y;
             const float z = DIM == 3 ? bz - pj.z : 0.0f;
             const float r2 = fmaf(z, z, fmaf(y, y, x * x));
-            if (r2 < S.rs2 && q != skip && (!bdy || pj.w != 0.0f)) {
+            if (r2 < S.rs2 && q != skip && pj.w >= wmin) {
               int qq = q;
               asm volatile("" : "+v"(qq));
               ap.add(nl_entry(jlo + qq, code));
@@ -935,7 +943,7 @@ template <int DIM, bool EXTRAP_DENSITY, bool NL>
 __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
-  if (!is_bdy(a.cur.type[p])) return;
+  if (!is_solid(a.cur.type[p])) return;  // boundary and rigid (muI :96, DP :221)
   const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
@@ -998,6 +1006,54 @@ __device__ __forceinline__ void soil_dv_pair(const SceneDev& S, const float* __r
   }
 }
 
+// d_vel of a dynamic rigid particle p of a soil scene: g (muI :111-112, DP
+// :233-234) minus the calc_d_vel_from_stress_task term of every soil
+// neighbour i (muI :38-46, DP :158-165), formed from i's side (x_ip, i's
+// corrected gradient) and gathered in p's neighbour order (the reference
+// subtracts them with atomic updates in no fixed order).
+template <int DIM, bool NL, int KT>
+__device__ __forceinline__ float4 soil_rigid_force(const KArgs& a, const SceneDev& S, int p) {
+  const Centre C = centre_of(a, S, p);
+  const int8_t* __restrict__ type = a.cur.type;
+  const float4* __restrict__ vtmp = a.cur.vtmp;
+  const float* __restrict__ sgt = a.w.sigt_f;
+  const int st = a.stride;
+  const float rtp = vtmp[p].w, mVp = C.P.w;
+  const bool kc = S.kcorr != 0;
+  float dv[3] = {S.grav[0], S.grav[1], S.grav[2]};
+  neighbours<DIM, NL>(S, a, p, C, [&](int i, float x, float y, float z, float r2, float, int) {
+    if (type[i] != kSoil) return;
+    const float f = kernel_gradfac_t<KT>(S, sqrtf(r2));
+    float gw[3] = {f * -x, f * -y, f * -z};
+    if (kc) {
+      float M[9];
+      corr_load(a.w.Lc, st, i, M);
+      corr_apply(M, gw);
+    }
+    const float ri2 = vtmp[i].w * vtmp[i].w;
+    float si[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) si[c] = sgt[c * st + i] / ri2;
+    float t[3] = {0.f, 0.f, 0.f};
+    soil_dv_pair<DIM>(S, sgt, st, p, si, mVp, rtp, gw, t);
+    dv[0] -= t[0];
+    dv[1] -= t[1];
+    dv[2] -= t[2];
+  });
+  return make_float4(dv[0], dv[1], dv[2], 0.0f);
+}
+// The non-soil branch of the soil force passes: zero rates, or the
+// gathered force of a dynamic rigid particle (a static one keeps 0,
+// muI :130-132, DP :272-274).
+template <int DIM, bool NL>
+__device__ __forceinline__ float4 soil_nonsoil_dv(const KArgs& a, int s, int p) {
+  const int tp = a.cur.type[p];
+  if (tp != kRigid) return make_float4(0.f, 0.f, 0.f, 0.f);
+  const SceneDev S = a.scenes[s];
+  if (rigid_slot(S, tp, a.cur.obj[p]) < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+  return S.kernel == 1 ? soil_rigid_force<DIM, NL, 1>(a, S, p) : soil_rigid_force<DIM, NL, 0>(a, S, p);
+}
+
 __device__ __forceinline__ float4 soil_dv_finish(const SceneDev& S, const float dv[3], float4 vi) {
   const float damp = -5e-5f * sqrtf(S.young / (vi.w * S.h2));
   return make_float4(dv[0] + S.grav[0] + damp * vi.x, dv[1] + S.grav[1] + damp * vi.y,
@@ -1009,7 +1065,7 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
   if (a.cur.type[p] != kSoil) {
-    a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.w.dv[p] = soil_nonsoil_dv<DIM, NL>(a, s, p);
     return;
   }
   const SceneDev S = a.scenes[s];
@@ -1248,7 +1304,7 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   if (a.cur.type[p] != kSoil) {
     const int st = a.stride;
     a.w.drho[p] = 0.0f;
-    a.w.dv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.w.dv[p] = soil_nonsoil_dv<DIM, NL>(a, s, p);
 #pragma unroll
     for (int c = 0; c < 6; ++c) a.w.dsig_d[c * st + p] = 0.0;
     a.w.dseq[p] = 0.0;

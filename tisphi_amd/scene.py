@@ -179,10 +179,6 @@ def build_scene(cfg, generate=True) -> Scene:
         raise NotImplementedError("OBJ bodies are not built")
 
     mat_index, fluids, soils, rigids = set_material(cfg.get_materials())
-    if method != METHOD_WC and rigids and any(
-            get_material(mat_index, (fluids, soils, rigids), b["materialId"])["matType"] == MAT_RIGID
-            for b in cfg.get_blocks()):
-        raise NotImplementedError("rigid blocks are built for WCSPH scenes only")
     if method == METHOD_WC:
         material = fluids[0]
         vsound = 60.0
