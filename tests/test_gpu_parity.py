@@ -46,6 +46,10 @@ CASES = {
     "wc2d_rigid": ("dam_break_2d", {"rigid": [(10, (1.205, 0.05, 0.0), (0.1, 0.1, 0.1), True),
                                               (11, (2.0, 0.0, 0.0), (0.1, 0.2, 0.1), False)]},
                    150, 0.6),
+    "wc3d_rigid": ("dam_break_3d", {"particleRadius": 0.01,
+                                    "rigid": [(10, (1.21, 0.05, 0.2), (0.1, 0.1, 0.1), True),
+                                              (11, (2.0, 0.0, 0.3), (0.1, 0.2, 0.1), False)]},
+                   30, 0.6),
     "mui2d_rigid": ("column_collapse_mui", {"rigid": [(10, (0.401, 0.005, 0.0), (0.02, 0.02, 0.02), True),
                                                       (11, (0.6, 0.0, 0.0), (0.01, 0.03, 0.02), False)]},
                     30, 0.2),
@@ -204,7 +208,7 @@ def test_single_pass_parity(case):
     np.testing.assert_array_equal(eng.download(slot, "id0"), o2.get("id0"))
     # the flow pushes the dynamic box: d_vel != g (the DP column's free face
     # is stress-free, so there the box feels next to nothing yet)
-    if key in ("wc2d_rigid", "mui2d_rigid"):
+    if key in ("wc2d_rigid", "wc3d_rigid", "mui2d_rigid"):
         t, obj = eng.download(slot, "type"), eng.download(slot, "obj_id")
         dv = eng.download(slot, "d_vel")[(t == 11) & (obj == 10)]
         assert np.max(np.abs(dv - np.array(sc.gravity))) > 0.1
