@@ -31,6 +31,42 @@ def add_rigid_blocks(cfg, blocks, density=500.0):
     return cfg
 
 
+def box_mesh(lo, hi):
+    """Vertices and quad faces (1-based, outward) of an axis-aligned box."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    v = np.array([[(hi if (k >> a) & 1 else lo)[a] for a in range(3)] for k in range(8)])
+    quads = [[5, 7, 3, 1], [4, 8, 6, 2], [2, 6, 5, 1], [7, 8, 4, 3], [3, 4, 2, 1], [6, 8, 7, 5]]
+    return v, quads
+
+
+def write_obj(path, verts, faces):
+    """A Wavefront OBJ file of ``verts`` and 1-based polygon ``faces``."""
+    with open(path, "w") as f:
+        f.write("# test mesh\n")
+        for x in verts:
+            f.write("v %.17g %.17g %.17g\n" % tuple(x))
+        for q in faces:
+            f.write("f " + " ".join(str(int(k)) for k in q) + "\n")
+    return str(path)
+
+
+def add_obj_bodies(cfg, bodies, density=500.0):
+    """Append a rigid material and OBJ bodies to a scene config:
+    ``bodies`` = [(objectId, path, translation, scale, angle_deg, axis,
+    isDynamic), ...]."""
+    c = cfg.config
+    mat_id = len(c["Materials"])
+    c["Materials"].append({"matId": mat_id, "matType": 11, "color": [200, 60, 60],
+                           "density0": density})
+    c.setdefault("Bodies", [])
+    for oid, path, tr, scale, angle, axis, dyn in bodies:
+        c["Bodies"].append({"objectId": oid, "geometryFile": path, "translation": list(tr),
+                            "scale": scale, "rotationAngle": angle, "rotationAxis": list(axis),
+                            "velocity": [0.0, 0.0, 0.0], "materialId": mat_id,
+                            "isDynamic": bool(dyn)})
+    return cfg
+
+
 def tiny_cfg(method=1, dim=2, xsph=False, r=None, domain=(0.2, 0.2, 0.06), block=(0.08, 0.08, 0.04),
              boundary=2, ti=2, kernel=1, kcorr=0):
     """A small dam-break / column scene for brute-force checks."""

@@ -11,11 +11,15 @@ every field) to the engine, and then compares pass by pass:
 Plus: fused step == pass sequence (bitwise), graph == eager (bitwise), and
 1000-step globals of the 2D dam break within 1% of the FP64 oracle.
 """
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
 from oracle.oracle import Oracle, load_scene
-from tests.common import TOL, add_rigid_blocks, parity_err, scene_cfg
+from tests.common import (TOL, add_obj_bodies, add_rigid_blocks, box_mesh, parity_err, scene_cfg,
+                          write_obj)
 from tisphi_amd import _lib as L
 from tisphi_amd.scene import build_scene
 
@@ -53,6 +57,12 @@ CASES = {
     "mui2d_rigid": ("column_collapse_mui", {"rigid": [(10, (0.401, 0.005, 0.0), (0.02, 0.02, 0.02), True),
                                                       (11, (0.6, 0.0, 0.0), (0.01, 0.03, 0.02), False)]},
                     30, 0.2),
+    # OBJ bodies: a dynamic box mesh turned 30 degrees beside the column and
+    # a static one turned 45 degrees (voxelised at the particle diameter)
+    "wc3d_body": ("dam_break_3d", {"particleRadius": 0.01,
+                                   "bodies": [(10, (1.25, 0.05, 0.2), 30.0, True),
+                                              (11, (2.0, 0.0, 0.3), 45.0, False)]},
+                  30, 0.6),
     "dp2d_rigid": ("column_collapse_dp", {"rigid": [(10, (0.401, 0.005, 0.0), (0.02, 0.02, 0.02), True),
                                                     (11, (0.6, 0.0, 0.0), (0.01, 0.03, 0.02), False)]},
                    30, 0.2),
@@ -71,6 +81,11 @@ def make_case(key):
         mod = dict(mod)
         if "rigid" in mod:
             add_rigid_blocks(cfg, mod.pop("rigid"))
+        if "bodies" in mod:
+            d = tempfile.mkdtemp(prefix="tsph_obj_")
+            path = write_obj(os.path.join(d, "box.obj"), *box_mesh((0, 0, 0), (0.1, 0.1, 0.1)))
+            add_obj_bodies(cfg, [(oid, path, tr, 1.0, ang, (0, 0, 1), dyn)
+                                 for oid, tr, ang, dyn in mod.pop("bodies")])
         if "domainEnd" in mod or "is2D" in mod:
             cfg.config["Blocks"][0]["size"] = [0.2, 0.1, 0.05]
         cfg.config["Configuration"].update(mod)
@@ -208,7 +223,7 @@ def test_single_pass_parity(case):
     np.testing.assert_array_equal(eng.download(slot, "id0"), o2.get("id0"))
     # the flow pushes the dynamic box: d_vel != g (the DP column's free face
     # is stress-free, so there the box feels next to nothing yet)
-    if key in ("wc2d_rigid", "wc3d_rigid", "mui2d_rigid"):
+    if key in ("wc2d_rigid", "wc3d_rigid", "mui2d_rigid", "wc3d_body"):
         t, obj = eng.download(slot, "type"), eng.download(slot, "obj_id")
         dv = eng.download(slot, "d_vel")[(t == 11) & (obj == 10)]
         assert np.max(np.abs(dv - np.array(sc.gravity))) > 0.1

@@ -14,6 +14,7 @@ class SimConfiger:
     def __init__(self, scene_file_path=None, config=None) -> None:
         """Load a scene from ``scene_file_path`` or take an already-parsed
         ``config`` dict (used by batched / served scenes generated in code)."""
+        self.scene_file_path = scene_file_path
         if config is not None:
             self.config = copy.deepcopy(config)
         else:

@@ -63,6 +63,19 @@ def _stress33(s6):
     return m
 
 
+def record_objects(blocks, bodies, sc):
+    """Object collection by object id, with each block's and body's
+    particle count in its "particleNum" (particle_system.py:72-91)."""
+    coll = {}
+    for b, n in zip(blocks, sc.block_counts):
+        b["particleNum"] = n
+        coll[b["objectId"]] = b
+    for b, n in zip(bodies, sc.body_counts):
+        b["particleNum"] = n
+        coll[b["objectId"]] = b
+    return coll
+
+
 class _ParticleAccessor:
     """``ps.pt.<member>`` -> FieldView of the current device state."""
 
@@ -126,12 +139,9 @@ class ParticleSystem:
             sc.mat_index, sc.mat_fluid, sc.mat_soil, sc.mat_rigid)
         self.blocks = config.get_blocks()
         self.bodies = config.get_bodies()
-        self.object_collection = {}
+        self.object_collection = record_objects(self.blocks, self.bodies, sc)
         self.object_id_rigid = {r["obj"] for r in sc.rigid}
         self.rigid_rest_cm = {r["obj"]: r["rest_cm"] for r in sc.rigid if r["dynamic"]}
-        for b, n in zip(self.blocks, sc.block_counts):
-            b["particleNum"] = n
-            self.object_collection[b["objectId"]] = b
         self.particle_max_num = sc.n
         self._vmax, self._vmin = 0.0, 0.0
         self._colour_inputs = False

@@ -6,8 +6,9 @@ constructors, done on the host in float64:
 * geometry  (A1)  — eng/particle_system.py:14-58: d = 2r, h = kh*d,
   r_s = kappa*h, grid = ceil(kappa*kh)*d, virtual domain = domain +- grid
   (2D: z extent d and grid_num[2] = 1), grid_num = ceil(vsize/grid).
-* particles (A2)  — eng/particle_system.py:72-108, 135-174, 274-314:
-  blocks in file order, then dummy walls, then the repulsive layer
+* particles (A2)  — eng/particle_system.py:72-108, 135-200, 274-314:
+  blocks in file order, then OBJ bodies (voxelised, eng/voxelize.py),
+  then dummy walls, then the repulsive layer
   (boundary 3 / 4: spacing r, particle_func.py:346-374); m_V = d^dim,
   mass = m_V*rho with rho = 0 for boundary particles; id0 = generation
   index (set_id0, :208-211).
@@ -104,6 +105,7 @@ class Scene:
     mat_soil: list
     mat_rigid: list
     block_counts: list = field(default_factory=list)
+    body_counts: list = field(default_factory=list)  # voxels per OBJ body
     n_dummy: int = 0
     n_rep: int = 0
     # host arrays, generation order
@@ -124,7 +126,7 @@ class Scene:
     @property
     def n(self):
         if self.pos is None:
-            return int(sum(self.block_counts) + self.n_dummy + self.n_rep)
+            return int(sum(self.block_counts) + sum(self.body_counts) + self.n_dummy + self.n_rep)
         return int(self.pos.shape[0])
 
     @property
@@ -175,8 +177,6 @@ def build_scene(cfg, generate=True) -> Scene:
     ti_flag = cfg.get_cfg("timeIntegration")
     if ti_flag not in (1, 2, 4):
         raise NotImplementedError(f"timeIntegration {ti_flag} is not built")
-    if cfg.get_bodies():
-        raise NotImplementedError("OBJ bodies are not built")
 
     mat_index, fluids, soils, rigids = set_material(cfg.get_materials())
     if method == METHOD_WC:
@@ -201,6 +201,24 @@ def build_scene(cfg, generate=True) -> Scene:
         chk_block_in_domain(geo.domain_start, geo.domain_end, b["translation"], b["size"], geo.dim)
         n, _ = calc_cube_particle_num(b["translation"], b["size"], geo.dim, offset=geo.particle_diameter)
         sc.block_counts.append(n)
+    bodies = cfg.get_bodies()
+    body_pts = []
+    if bodies and geo.dim != 3:
+        # a voxelised mesh is a 3D point set; the reference has no 2D form
+        raise NotImplementedError("OBJ bodies need a 3D scene")
+    if bodies:
+        import os
+        from .eng.voxelize import load_body
+        path = getattr(cfg, "scene_file_path", None)
+        base = os.path.dirname(os.path.abspath(path)) if path else None
+        for b in bodies:
+            # voxelised at the particle diameter (particle_system.py:82-91)
+            body_pts.append(load_body(b, geo.particle_diameter, base))
+            p = body_pts[-1]
+            if p.size and (np.any(p < np.asarray(geo.domain_start)) or np.any(p > np.asarray(geo.domain_end))):
+                # the reference would index its cell grid out of range
+                raise ValueError(f"body {b['objectId']} leaves the domain")
+            sc.body_counts.append(int(body_pts[-1].shape[0]))
     dummy = None
     if boundary in (BDY_DUMMY, BDY_DUMMY_REP):
         dummy = calc_dummy_boundary(geo.dim, geo.domain_start, geo.domain_end,
@@ -230,6 +248,19 @@ def build_scene(cfg, generate=True) -> Scene:
         oid.append(np.full(n, b["objectId"], dtype=np.int32))
         mid.append(np.full(n, b["materialId"], dtype=np.int32))
         col[-1] = np.tile(col[-1], (n, 1))
+    for b, p in zip(bodies, body_pts):
+        # init_body (particle_system.py:176-200): after every block, before
+        # the boundary particles
+        mat = get_material(mat_index, lists, b["materialId"])
+        n = p.shape[0]
+        pos.append(p)
+        vel.append(np.tile(np.array(b["velocity"], dtype=np.float64), (n, 1)))
+        rho.append(np.full(n, float(mat["density0"])))
+        mtype.append(np.full(n, mat["matType"], dtype=np.int32))
+        oid.append(np.full(n, b["objectId"], dtype=np.int32))
+        mid.append(np.full(n, b["materialId"], dtype=np.int32))
+        col.append(np.tile(np.array([ic / 255 for ic in mat.get("color", [0, 0, 0])],
+                                    dtype=np.float32), (n, 1)))
     if dummy is not None:
         for lo, hi in dummy:
             p = cube_positions(lo, hi - lo, geo.dim, geo.particle_diameter)
@@ -263,7 +294,7 @@ def build_scene(cfg, generate=True) -> Scene:
     sc.color0 = np.concatenate(col) if col else np.zeros((0, 3), np.float32)
     sc.m_V = np.full(sc.n, geo.m_V0)
     sc.mass = geo.m_V0 * sc.density
-    for b in blocks:
+    for b in list(blocks) + list(bodies):
         mat = get_material(mat_index, lists, b["materialId"])
         if mat["matType"] != MAT_RIGID:
             continue
