@@ -94,12 +94,11 @@ def test_escaped_particle_is_reported_not_faulted():
     sc = build_scene(scene_cfg("dam_break_2d"))
     e = L.Engine(sc.method, sc.geo.dim, sc.n, sc.geo.grid_num_total, 1)
     s = e.add_scene(sc.desc())
-    pos = sc.pos.copy()
-    pos[10] = [5.0, 0.3, 0.0]  # beyond the virtual domain: the hash clamps and counts it
-    sc2 = sc
-    sc2.pos = pos
-    L.upload_scene(e, s, sc2)
-    e.step(1)
+    L.upload_scene(e, s, sc)
+    vel = sc.vel.copy()
+    vel[10] = [1.0e5, 0.0, 0.0]  # 10 m in one step: the next hash clamps and counts it
+    e.upload(s, "vel", vel)
+    e.step(2)
     with pytest.raises(L.ParticleEscaped):
         e.sync()
     assert e.status(s)[0] >= 1

@@ -114,3 +114,31 @@ def test_capacity_limits_are_errors():
     eng.close()
     with pytest.raises(ValueError):
         L.Engine(sc.method, 2, (1 << 26) + 1, 100, 1)
+
+
+def test_out_of_domain_inputs_are_argument_errors():
+    """A position outside the cell grid over the virtual domain (the
+    reference's pos_to_index would address a cell that does not exist) or
+    a cell id out of range is refused at upload; the engine stays usable."""
+    sc = build_scene(tiny_cfg())
+    eng = L.Engine(sc.method, 2, sc.n, sc.geo.grid_num_total, 1)
+    slot = eng.add_scene(sc.desc())
+    L.upload_scene(eng, slot, sc)
+    bad = sc.pos.copy()
+    bad[5, 0] = sc.geo.vdomain_start[0] - 1e-3
+    with pytest.raises(ValueError):
+        eng.upload(slot, "pos", bad)
+    bad = sc.pos.copy()
+    g = sc.geo
+    bad[7, 1] = g.vdomain_start[1] + g.grid_num[1] * g.grid_size + 1e-3  # past the last cell
+    with pytest.raises(ValueError):
+        eng.upload(slot, "pos", bad)
+    cell = eng.download(slot, "cell")
+    cell[3] = sc.geo.grid_num_total
+    with pytest.raises(ValueError):
+        eng.upload(slot, "cell", cell)
+    L.upload_scene(eng, slot, sc)
+    eng.step(3)
+    eng.sync()
+    assert eng.status(slot) == (0, 0)
+    eng.close()

@@ -752,7 +752,12 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
               continue;
             }
             const double rel = x - S.vds[ax];
-            const int ci = (int)(rel / S.grid);  // pos_to_index, particle_system.py:216-218
+            const double cf = rel / S.grid;
+            // the cell grid covers the virtual domain; outside it the
+            // reference's pos_to_index would address cells that do not exist
+            if (!(cf >= 0.0 && cf < (double)S.gn[ax]))
+              return fail(TSPH_ERR_INVALID_ARG, "particle " + std::to_string(i) + " lies outside the cell grid");
+            const int ci = (int)cf;  // pos_to_index, particle_system.py:216-218
             c[ax] = ci;
             e[ax] = (float)(rel - (double)ci * S.grid);
           } else {
@@ -886,7 +891,12 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
     }
     case TSPH_F_ID0: return i1(cur.id0);
     case TSPH_F_OBJ_ID: return i1(cur.obj);
-    case TSPH_F_CELL: return i1(cur.cell);
+    case TSPH_F_CELL:
+      if (up)
+        for (int64_t i = 0; i < n; ++i)
+          if (hi[i] < 0 || hi[i] >= S.ncell)
+            return fail(TSPH_ERR_INVALID_ARG, "cell id " + std::to_string(hi[i]) + " out of range");
+      return i1(cur.cell);
     case TSPH_F_FLAG_RETMAP:
       if (!dp) {
         if (up) return TSPH_OK;
