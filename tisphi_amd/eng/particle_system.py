@@ -161,6 +161,20 @@ class ParticleSystem:
         """Upload the generated lattice (blocks, then dummy walls)."""
         _lib.upload_scene(self.engine, self.slot, self.scene)
 
+    def save_state(self, path, step=0):
+        """Write this scene's state after ``step`` steps to an ``.npz``
+        snapshot (an engine addition: the reference has no checkpoint)."""
+        from .. import snapshot
+        return snapshot.save(path, self.engine, self.slot, self.scene, step)
+
+    def load_state(self, path):
+        """Replace this scene's state by a snapshot of the same scene;
+        returns the snapshot's header (step count, dt, config)."""
+        from .. import snapshot
+        meta, arrays = snapshot.load(path)
+        snapshot.restore(self.engine, self.slot, meta, arrays)
+        return meta
+
     def set_id0(self):
         self.engine.upload(self.slot, "id0", np.arange(self.particle_max_num, dtype=np.int32))
 

@@ -15,6 +15,19 @@ class Simulation:
         self.ps = ParticleSystem(self.cfg)
         self.solver = self.build_solver()
 
+    @classmethod
+    def resume(cls, path):
+        """(simulation, step) continued from a snapshot written by
+        ``ps.save_state``: the scene is rebuilt from the JSON the snapshot
+        holds, then its state is uploaded over the generated one."""
+        from .. import snapshot
+        meta, arrays = snapshot.load(path)
+        cfg = SimConfiger(config=meta["config"])
+        cfg.scene_file_path = meta.get("scene_file")
+        case = cls(cfg)
+        snapshot.restore(case.ps.engine, case.ps.slot, meta, arrays)
+        return case, int(meta["step"])
+
     def build_solver(self):
         if self.solver_type == 1:
             return WCSPHSolver(self.ps)

@@ -10,6 +10,8 @@ end the run (``cur_time = dt * count_step``), ``stopAtStep`` /
 ignored, exports follow ``exportEveryTime`` / ``exportEveryRender`` with
 ``exportCSV`` / ``exportVTK`` (PNG frames need the window and are not
 written).  ``--steps`` caps the run for scenes that set neither exit rule.
+``--checkpoint`` writes a state snapshot at the end and ``--resume``
+continues one (an engine addition; the reference has no checkpoint).
 As the window loop, the run colours the particles at step 0 and after every
 frame (``assign_color``, eng/ui_sim.py:94, 107, 282-286) when colorTitle > 0,
 on the device: ``case.ps.pt.val`` / ``case.ps.pt.color`` hold the result.
@@ -33,9 +35,17 @@ def assign_color(case, given_max, given_min, fix_max, fix_min):
         case.ps.set_color()
 
 
-def run(scene_path, max_steps=None, out_dir=None, log=print):
-    cfg = SimConfiger(scene_file_path=scene_path)
-    case = Simulation(cfg)
+def run(scene_path, max_steps=None, out_dir=None, log=print, resume=None, checkpoint=None):
+    """Run a scene (or continue the snapshot ``resume``); ``max_steps``
+    counts from step 0 like exitAtStep; ``checkpoint`` names a snapshot
+    written at the end."""
+    if resume:
+        case, count_step = Simulation.resume(resume)
+        cfg = case.cfg
+    else:
+        cfg = SimConfiger(scene_file_path=scene_path)
+        case = Simulation(cfg)
+        count_step = 0
     g = cfg.get_cfg_default
     substeps = max(1, int(g("stepsPerRenderUpdate", 1)))
     exit_at_step = g("exitAtStep", 0)
@@ -52,8 +62,10 @@ def run(scene_path, max_steps=None, out_dir=None, log=print):
     if not (exit_at_step or exit_at_time or max_steps):
         raise SystemExit("scene sets neither exitAtStep nor exitAtTime: pass --steps")
     dt = case.solver.dt[None]
-    count_step = 0
+    start_step = count_step
     next_time = every_time
+    while every_time > 0 and next_time <= dt * count_step:
+        next_time += every_time  # thresholds a resumed run already passed
     colour = (g("givenMax", -1), g("givenMin", -1), g("fixMax", 0), g("fixMin", 0))
     assign_color(case, *colour)
 
@@ -64,12 +76,14 @@ def run(scene_path, max_steps=None, out_dir=None, log=print):
         if g("exportVTK", False):
             export.export_vtu(os.path.join(out_dir, f"sim.vtk.{stamp}"), pos, data)
 
-    if judge_path and every_time > 0:
+    if judge_path and every_time > 0 and not resume:
         do_export(export.time_stamp(0.0))
-    elif judge_path and every_render > 0:
+    elif judge_path and every_render > 0 and not resume:
         do_export(export.step_stamp(0))
     t0 = time.perf_counter()
-    while True:
+    done = ((exit_at_step and count_step >= exit_at_step) or (exit_at_time and dt * count_step >= exit_at_time)
+            or (max_steps and count_step >= max_steps))
+    while not done:
         n = substeps
         if max_steps:
             n = min(n, max_steps - count_step)
@@ -83,14 +97,15 @@ def run(scene_path, max_steps=None, out_dir=None, log=print):
                 next_time += every_time
             elif every_render > 0 and count_step % (every_render * substeps) == 0:
                 do_export(export.step_stamp(count_step))
-        if ((exit_at_step and count_step >= exit_at_step) or (exit_at_time and cur_time >= exit_at_time)
-                or (max_steps and count_step >= max_steps)):
-            break
+        done = ((exit_at_step and count_step >= exit_at_step) or (exit_at_time and cur_time >= exit_at_time)
+                or (max_steps and count_step >= max_steps))
     case.solver.sync()
     wall = time.perf_counter() - t0
     glb = case.ps.globals()
-    log(f"steps {count_step}  sim time {dt * count_step:.6f}s  wall {wall:.3f}s  "
-        f"front {glb[0]:.6f}  PE {glb[1]:.6f}  KE {glb[2]:.6f}  M {glb[3]:.6f}")
+    log(f"steps {count_step} ({count_step - start_step} run)  sim time {dt * count_step:.6f}s  "
+        f"wall {wall:.3f}s  front {glb[0]:.6f}  PE {glb[1]:.6f}  KE {glb[2]:.6f}  M {glb[3]:.6f}")
+    if checkpoint:
+        case.ps.save_state(checkpoint, count_step)
     return case, count_step, glb
 
 
@@ -99,8 +114,12 @@ def main(argv=None):
     ap.add_argument("--scene_file", default="", help="scene file")
     ap.add_argument("--steps", type=int, default=None, help="stop after this many steps")
     ap.add_argument("--out", default=None, help="export directory")
+    ap.add_argument("--resume", default=None, help="continue from this state snapshot (.npz)")
+    ap.add_argument("--checkpoint", default=None, help="write a state snapshot (.npz) at the end")
     a = ap.parse_args(argv)
-    run(a.scene_file, a.steps, a.out)
+    if not a.scene_file and not a.resume:
+        ap.error("--scene_file or --resume is needed")
+    run(a.scene_file, a.steps, a.out, resume=a.resume, checkpoint=a.checkpoint)
     return 0
 
 
