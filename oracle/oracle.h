@@ -45,6 +45,10 @@ int orc_globals(void* h, double out[4]);
  * on the stored representation); returns the count, writes up to cap. */
 int64_t orc_neighbors(void* h, int64_t i, int64_t* out, int64_t cap);
 int64_t orc_escaped(void* h);
+/* Dynamic rigid object (shape matching) with its rest centre of mass. */
+int orc_set_rigid_object(void* h, int obj, const double cm[3]);
+/* Orthogonal polar factor R of a row-major 3x3 A (dim 2: the xy block). */
+int orc_polar(const double* A, int dim, double* R);
 void orc_set_threads(int n);
 /* Function-level hooks for closed-form tests. */
 int orc_kernel_eval(const orc_params* p, int precision, const double* r, int64_t n, double* W,
