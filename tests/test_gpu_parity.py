@@ -63,6 +63,15 @@ CASES = {
                                    "bodies": [(10, (1.25, 0.05, 0.2), 30.0, True),
                                               (11, (2.0, 0.0, 0.3), 45.0, False)]},
                   30, 0.6),
+    "mui3d_rigid": ("column_collapse_mui", {"is2D": False, "particleRadius": 0.004,
+                                            "domainEnd": [0.6, 0.3, 0.1],
+                                            "rigid": [(10, (0.201, 0.005, 0.02), (0.024, 0.024, 0.024), True)]},
+                    20, 0.2),
+    "dp3d_rigid": ("column_collapse_dp", {"is2D": False, "particleRadius": 0.004,
+                                          "domainEnd": [0.6, 0.3, 0.1],
+                                          "rigid": [(10, (0.201, 0.005, 0.02), (0.024, 0.024, 0.024), True),
+                                                    (11, (0.4, 0.0, 0.02), (0.016, 0.04, 0.024), False)]},
+                   20, 0.2),
     "dp2d_rigid": ("column_collapse_dp", {"rigid": [(10, (0.401, 0.005, 0.0), (0.02, 0.02, 0.02), True),
                                                     (11, (0.6, 0.0, 0.0), (0.01, 0.03, 0.02), False)]},
                    30, 0.2),
@@ -223,7 +232,7 @@ def test_single_pass_parity(case):
     np.testing.assert_array_equal(eng.download(slot, "id0"), o2.get("id0"))
     # the flow pushes the dynamic box: d_vel != g (the DP column's free face
     # is stress-free, so there the box feels next to nothing yet)
-    if key in ("wc2d_rigid", "wc3d_rigid", "mui2d_rigid", "wc3d_body"):
+    if key in ("wc2d_rigid", "wc3d_rigid", "mui2d_rigid", "mui3d_rigid", "wc3d_body"):
         t, obj = eng.download(slot, "type"), eng.download(slot, "obj_id")
         dv = eng.download(slot, "d_vel")[(t == 11) & (obj == 10)]
         assert np.max(np.abs(dv - np.array(sc.gravity))) > 0.1
