@@ -69,6 +69,23 @@ def test_round_trip(tmp_path, name):
         assert b.rigid == {10: [float(v) for v in sc.rigid[0]["rest_cm"]]}
 
 
+def test_scene_json_drops_derived_keys(tmp_path):
+    """Loader-derived keys (particle counts, a dynamic body's rest mesh)
+    stay out of the header; the scene JSON itself is kept."""
+    from tests.common import add_obj_bodies, box_mesh, write_obj
+    cfg = scene_cfg("dam_break_3d")
+    cfg.config["Configuration"]["particleRadius"] = 0.01
+    path = write_obj(tmp_path / "b.obj", *box_mesh((0, 0, 0), (0.1, 0.1, 0.1)))
+    add_obj_bodies(cfg, [(7, path, (1.5, 0.1, 0.2), 1.0, 0.0, (0, 0, 1), True)])
+    sc = build_scene(cfg)
+    assert "restPosition" in cfg.get_bodies()[0]
+    p = snapshot.save(str(tmp_path / "s.npz"), FakeEngine(sc), 0, sc)
+    meta, _ = snapshot.load(p)
+    body = meta["config"]["Bodies"][0]
+    assert "restPosition" not in body and body["geometryFile"] == path
+    assert meta["config"]["Configuration"] == cfg.config["Configuration"]
+
+
 def test_restore_checks(tmp_path):
     sc = build_scene(scene_cfg("dam_break_2d"))
     path = snapshot.save(str(tmp_path / "s.npz"), FakeEngine(sc), 0, sc)

@@ -175,6 +175,10 @@ def test_scene_with_body(tmp_path):
     cfg = body_scene(tmp_path)
     want = load_body(cfg.get_bodies()[0], sc.geo.particle_diameter)
     np.testing.assert_array_equal(sc.pos[body], want)
+    # a dynamic body keeps its rest mesh, as the reference's load_body does
+    b = cfg.get_bodies()[0]
+    assert b["restPosition"].shape == (8, 3)
+    np.testing.assert_allclose(b["restCenterOfMass"], b["restPosition"].mean(axis=-1))
     # a static body is a rigid object without shape matching
     sc2 = build_scene(body_scene(tmp_path, dyn=False))
     assert [x["dynamic"] for x in sc2.rigid] == [False]

@@ -110,4 +110,11 @@ def load_body(body, vox_len, base_dir=None):
         point = v.mean(axis=-1)[:3]
         M = rotation_matrix(angle, body.get("rotationAxis", [0, 0, 1]), point)
         v = v @ M[:3, :3].T + M[:3, 3]
-    return voxel_points(v, f, vox_len) + np.asarray(body["translation"], dtype=np.float64)
+    offset = np.asarray(body["translation"], dtype=np.float64)
+    if body.get("isDynamic", False):
+        # the rest mesh the reference keeps on a dynamic body
+        # (particle_func.py:278-284; "restCenterOfMass" is its per-vertex
+        # mean, mesh.vertices.mean(axis=-1), as there)
+        body["restPosition"] = v + offset
+        body["restCenterOfMass"] = body["restPosition"].mean(axis=-1)
+    return voxel_points(v, f, vox_len) + offset

@@ -38,6 +38,20 @@ def state_fields(method, dynamic_rigid):
     return f
 
 
+# keys the loaders add to Blocks / Bodies entries (particle counts, the
+# rest mesh of a dynamic body); rebuilt when the scene is built again
+DERIVED = ("particleNum", "restPosition", "restCenterOfMass", "voxelizedPoints", "mesh")
+
+
+def scene_json(config):
+    """The scene JSON without the keys the loaders derive from it."""
+    out = dict(config)
+    for sec in ("Blocks", "Bodies"):
+        if sec in out:
+            out[sec] = [{k: v for k, v in b.items() if k not in DERIVED} for b in out[sec]]
+    return out
+
+
 def save(path, eng, slot, scene, step=0):
     """Write the state of ``slot`` (built from ``scene``) after ``step``
     steps to ``path``; returns the path written."""
@@ -47,7 +61,7 @@ def save(path, eng, slot, scene, step=0):
     eng.sync()
     arrays = {f: eng.download(slot, f) for f in state_fields(scene.method, dyn)}
     meta = dict(format=FORMAT, method=int(scene.method), dim=int(scene.geo.dim), n=int(scene.n),
-                step=int(step), dt=float(scene.dt), rigid=rigid, config=scene.cfg.config,
+                step=int(step), dt=float(scene.dt), rigid=rigid, config=scene_json(scene.cfg.config),
                 scene_file=getattr(scene.cfg, "scene_file_path", None))
     header = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     with open(path, "wb") as f:
