@@ -746,9 +746,11 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
 template <int DIM, bool NL, bool SHEP, int KT, bool KC>
 __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, int p) {
   const Centre C = centre_of(a, S, p);
-  // A WC scene holds fluid and boundary particles only (checked at upload):
-  // a neighbour's pair record says which one it is, so the pass reads
-  // position + m_V and v_tmp + p/rho_tmp^2 per neighbour and nothing else.
+  // A WC scene holds fluid, boundary and rigid particles only (checked at
+  // upload), and rigid ones are walls to the fluid: a neighbour's pair
+  // record (written by the EOS or the boundary pass, sign bit set for a
+  // wall) says which it is, so the pass reads position + m_V and
+  // v_tmp + p/rho_tmp^2 per neighbour and nothing else.
   const float4* __restrict__ vp = a.w.vp;
   const float4 vi = a.cur.vtmp[p];
   const float pri = vp[p].w;
