@@ -46,6 +46,17 @@ def test_obj_reader(tmp_path):
     assert f.tolist() == [[0, 1, 2], [0, 2, 3], [0, 2, 3]]
 
 
+def test_degenerate_inputs(tmp_path):
+    p = tmp_path / "empty.obj"
+    p.write_text("v 0 0 0\nv 1 0 0\n")
+    v, f = load_obj(str(p))
+    with pytest.raises(ValueError):
+        voxel_points(v, f, 0.1)
+    v, f = tri(*box_mesh((0, 0, 0), (1, 1, 1)))
+    with pytest.raises(ValueError):
+        voxel_points(v, f, 0.0)
+
+
 def test_axis_aligned_box_is_its_lattice():
     for lo, hi, pitch in (((0, 0, 0), (1, 1, 1), 0.1), ((0, 0, 0), (0.5, 0.3, 0.2), 0.1),
                           ((-0.2, 0.1, 0.3), (0.2, 0.5, 0.5), 0.05)):

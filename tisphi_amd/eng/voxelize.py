@@ -83,6 +83,10 @@ def _surface_samples(v, faces, max_edge):
 
 def voxel_points(vertices, faces, pitch):
     """Centres of the filled voxelisation of a closed mesh at ``pitch``."""
+    if len(faces) == 0:
+        raise ValueError("the body mesh has no faces")
+    if not pitch > 0.0:
+        raise ValueError(f"voxel pitch must be positive, got {pitch}")
     pts = _surface_samples(vertices, faces, pitch / 2.0)
     hit = np.round(pts / pitch).astype(np.int64)
     lo = hit.min(axis=0)
