@@ -180,7 +180,9 @@ int tsph_engine_bytes(void* engine, int64_t* out_bytes);
 /* ---- scenes ----------------------------------------------------------- */
 /* Reserve a slot and its particle/cell ranges; fields start at zero.
  * Replaces Simulation.__init__ / build_solver (eng/simulation.py:8-22): the
- * solver type and its constants come from desc. */
+ * solver type and its constants come from desc.  The scene's method must be
+ * the engine's and its time integration that of the resident scenes (one
+ * fused step sequence per engine): TSPH_ERR_INVALID_ARG otherwise. */
 int tsph_add_scene(void* engine, const tsph_scene_desc* desc, int* out_slot);
 /* Free a slot (no reference counterpart: the reference runs one scene per
  * process). */

@@ -102,6 +102,10 @@ def test_capacity_limits_are_errors():
     d["method"] = 2
     with pytest.raises(ValueError):
         eng.add_scene(d)  # another method: an argument error
+    d = sc.desc()
+    d["time_integration"] = 1
+    with pytest.raises(ValueError):
+        eng.add_scene(d)  # another integrator than the resident scene's
     L.upload_scene(eng, slot, sc)
     eng.step(3)
     eng.sync()

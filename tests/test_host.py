@@ -4,8 +4,8 @@ import numpy as np
 import pytest
 
 from tisphi_amd import export
-from tisphi_amd.batch import build_family, dam_break_family
-from tisphi_amd.serve import plan_capacity, request_mix
+from tisphi_amd.batch import BatchEngine, build_family, dam_break_family
+from tisphi_amd.serve import Request, class_key, plan_capacity, request_mix
 
 
 def _dump(n=50, seed=0):
@@ -66,5 +66,19 @@ def test_request_mix_sizes_and_arrivals():
         assert 0.7 * lo <= nf <= 1.3 * hi, (r.kind, nf)
         assert 500 <= r.steps <= 5000
     cap = plan_capacity(reqs)
-    for (m, d), (p, c) in cap.items():
-        assert p >= max(r.scene.n for r in reqs if (r.scene.method, r.scene.geo.dim) == (m, d))
+    for k, (p, c) in cap.items():
+        assert p >= max(r.scene.n for r in reqs if class_key(r.scene) == k)
+
+
+def test_one_integrator_per_engine_class():
+    """An engine steps its residents with one fused sequence, so scenes of
+    another time integration go to another serving class and a batch
+    refuses them before any engine exists."""
+    from tests.common import tiny_cfg
+    from tisphi_amd.scene import build_scene
+    lf, se = build_scene(tiny_cfg(ti=2)), build_scene(tiny_cfg(ti=1))
+    assert class_key(lf) != class_key(se) and class_key(lf)[:2] == class_key(se)[:2]
+    cap = plan_capacity([Request(0, lf, 10, 0.0), Request(1, se, 10, 0.0)])
+    assert set(cap) == {class_key(lf), class_key(se)}
+    with pytest.raises(ValueError):
+        BatchEngine([lf, se])

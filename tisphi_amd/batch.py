@@ -39,14 +39,15 @@ def dam_break_family(n=256, seed=0, base="dam_break_2d"):
 
 
 class BatchEngine:
-    """Scenes of one method/dimension resident in one engine."""
+    """Scenes of one method, dimension and time integration resident in one
+    engine."""
 
     def __init__(self, scenes, spare_particles=0, spare_cells=0, max_scenes=None):
         if not scenes:
             raise ValueError("no scenes")
-        m, d = scenes[0].method, scenes[0].geo.dim
-        if any(s.method != m or s.geo.dim != d for s in scenes):
-            raise ValueError("a batch holds scenes of one method and dimension")
+        m, d, ti = scenes[0].method, scenes[0].geo.dim, scenes[0].time_integration
+        if any(s.method != m or s.geo.dim != d or s.time_integration != ti for s in scenes):
+            raise ValueError("a batch holds scenes of one method, dimension and time integration")
         npart = sum(s.n for s in scenes) + spare_particles
         ncell = sum(s.geo.grid_num_total for s in scenes) + spare_cells
         self.engine = _lib.Engine(m, d, npart, ncell, max_scenes or len(scenes))

@@ -570,6 +570,10 @@ int upload_tables(tsph_engine* E) {
 int validate_desc(const tsph_engine* E, const tsph_scene_desc* d) {
   if (!d) return fail(TSPH_ERR_INVALID_ARG, "null scene description");
   if (d->method != E->method) return fail(TSPH_ERR_INVALID_ARG, "scene method differs from the engine's");
+  // one fused step sequence per engine: its residents share the integrator
+  for (const SceneDev& o : E->scenes)
+    if (o.active && o.ti != d->time_integration)
+      return fail(TSPH_ERR_INVALID_ARG, "scene time integration differs from the resident scenes'");
   if (d->dim != E->dim) return fail(TSPH_ERR_INVALID_ARG, "scene dimension differs from the engine's");
   if (d->boundary < 0 || d->boundary > 4) return fail(TSPH_ERR_INVALID_ARG, "boundary must be 0..4");
   if (d->kernel != 0 && d->kernel != 1) return fail(TSPH_ERR_INVALID_ARG, "kernel must be 0 or 1");

@@ -1,7 +1,8 @@
 """Continuous-batching scene server.
 
 Requests (a scene plus a step budget) arrive over time.  The server keeps
-one engine per (method, dimension) class, each sized from an HBM budget and
+one engine per (method, dimension, time integration) class, each sized from
+an HBM budget and
 driven on its own HIP stream.  Between iterations it admits waiting requests
 into free scene slots / particle / cell ranges (first come, first served per
 class), advances every engine that has residents, and retires scenes whose
@@ -26,6 +27,12 @@ from .eng.configer_builder import SimConfiger
 from .scene import build_scene
 
 SCENES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "scenes")
+
+
+def class_key(scene):
+    """Engine class of a scene: an engine steps scenes of one method,
+    dimension and time integration."""
+    return (scene.method, scene.geo.dim, scene.time_integration)
 
 
 @dataclass
@@ -86,8 +93,8 @@ class _Class:
 
 class Server:
     def __init__(self, capacity, chunk=100, max_scenes=256):
-        """``capacity``: {(method, dim): (particles, cells)} — the HBM budget
-        split per class (see ``plan_capacity``)."""
+        """``capacity``: {(method, dim, time integration): (particles,
+        cells)} — the HBM budget split per class (see ``plan_capacity``)."""
         self.chunk = chunk
         self.classes = {k: _Class(k[0], k[1], p, c, max_scenes) for k, (p, c) in capacity.items()}
 
@@ -122,7 +129,7 @@ class Server:
             tn = now() if realtime else float("inf")
             while pending and pending[0].arrival <= tn:
                 r = pending.pop(0)
-                waiting[(r.scene.method, r.scene.geo.dim)].append(r)
+                waiting[class_key(r.scene)].append(r)
             for k, cls in self.classes.items():
                 q = waiting[k]
                 while q and self._admit(cls, q[0], now()):
@@ -173,7 +180,7 @@ def plan_capacity(requests, hbm_budget_bytes=None, headroom=1.0):
     bounded by an HBM budget split in proportion to particle counts."""
     need, big = {}, {}
     for r in requests:
-        k = (r.scene.method, r.scene.geo.dim)
+        k = class_key(r.scene)
         p, c = need.get(k, (0, 0))
         need[k] = (p + r.scene.n, c + r.scene.geo.grid_num_total)
         bp, bc = big.get(k, (0, 0))
