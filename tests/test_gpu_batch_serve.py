@@ -149,3 +149,22 @@ def test_facade_and_headless_cli(tmp_path):
     assert arr.shape == (9144, 17)
     pts, d = export.read_vtu_points(str(tmp_path / "out" / "sim.vtk.000010.vtu"))
     np.testing.assert_allclose(pts[:, 0], arr[:, 3], rtol=0, atol=1e-15)
+
+
+def test_mixed_options_batch_alone_equals_in_mix():
+    """Scenes with different per-scene switches (kernel, correction, XSPH,
+    boundary) in one engine: each ends bitwise where it ends alone, although
+    the engine's step sequence is the one of the union of the switches."""
+    opts = [dict(), dict(xsph=True), dict(kernel=0), dict(kcorr=1), dict(kcorr=2),
+            dict(boundary=1), dict(boundary=3), dict(boundary=4, xsph=True, kernel=0)]
+    scenes = [build_scene(tiny_cfg(**o)) for o in opts]
+    b = BatchEngine(scenes)
+    b.step(30)
+    b.sync()
+    for k, sc in enumerate(scenes):
+        e, s = alone(sc, 30)
+        for f in FIELDS_WC:
+            np.testing.assert_array_equal(b.engine.download(b.slots[k], f), e.download(s, f),
+                                          err_msg=f"{opts[k]}: {f}")
+        e.close()
+    b.close()
