@@ -153,10 +153,11 @@ def test_facade_and_headless_cli(tmp_path):
 
 def test_mixed_options_batch_alone_equals_in_mix():
     """Scenes with different per-scene switches (kernel, correction, XSPH,
-    boundary) in one engine: each ends bitwise where it ends alone, although
+    boundary, particle size) in one engine: each ends bitwise where it ends alone, although
     the engine's step sequence is the one of the union of the switches."""
     opts = [dict(), dict(xsph=True), dict(kernel=0), dict(kcorr=1), dict(kcorr=2),
-            dict(boundary=1), dict(boundary=3), dict(boundary=4, xsph=True, kernel=0)]
+            dict(boundary=1), dict(boundary=3), dict(boundary=4, xsph=True, kernel=0),
+            dict(r=0.004)]  # another particle size: its own dt, grid and cell range
     scenes = [build_scene(tiny_cfg(**o)) for o in opts]
     b = BatchEngine(scenes)
     b.step(30)
