@@ -7,6 +7,8 @@
 // sized from the engine capacity, so one captured hipGraph serves any set
 // of resident scenes.
 #pragma once
+#include <type_traits>
+
 #include "tsph_device.h"
 
 namespace tsph {
@@ -567,24 +569,36 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
           if (r2 < S.rs2 && j != p) rec(j, x, y, z, r2, pj.w, code);
         };
         if (staged) {
-          // One induction variable (the LDS slot); the particle index is
-          // formed only on a hit, from an opaque copy of the slot, so the
-          // compiler keeps no per-candidate index, entry or type pointer.
-          // p is a candidate of the own cell only (wave-uniform offset 0).
-          const int32_t skip = (dx == 0 && dy == 0 && dz == 0) ? p - jlo : -1;
+          // One induction variable, the LDS byte offset, kept opaque so
+          // the compiler derives no second one from the own-cell test; the
+          // particle index is formed only on a hit.  p is a candidate of the
+          // own cell only (wave-uniform offset 0), so only that cell's loop
+          // carries the test, and the record is read whole (one 16-byte LDS
+          // read) so the flow flag costs no second read on the hit path:
+          // 10 vector instructions per rejected candidate instead of 12.
+          const uint32_t oend = (uint32_t)(je - jlo) * 16u;
+          const char* __restrict__ sb = reinterpret_cast<const char*>(sp);
+          auto scan = [&](uint32_t skip, auto own) {
 #pragma unroll 1
-          for (int32_t q = jb - jlo; q < je - jlo; ++q) {
-            const float4 pj = sp[q];
-            const float x = bx - pj.x;
-            const float y = by - pj.y;
-            const float z = DIM == 3 ? bz - pj.z : 0.0f;
-            const float r2 = fmaf(z, z, fmaf(y, y, x * x));
-            if (r2 < S.rs2 && q != skip && pj.w >= wmin) {
-              int qq = q;
-              asm volatile("" : "+v"(qq));
-              ap.add(nl_entry(jlo + qq, code));
+            for (uint32_t o = (uint32_t)(jb - jlo) * 16u; o < oend; o += 16u) {
+              asm volatile("" : "+v"(o));
+              float4 pj = *reinterpret_cast<const float4*>(sb + o);
+              asm volatile("" : "+v"(pj.w));  // one 16-byte read per candidate
+              const float x = bx - pj.x;
+              const float y = by - pj.y;
+              const float z = DIM == 3 ? bz - pj.z : 0.0f;
+              const float r2 = fmaf(z, z, fmaf(y, y, x * x));
+              if (r2 < S.rs2 && (!decltype(own)::value || o != skip) && pj.w >= wmin) {
+                uint32_t oo = o;
+                asm volatile("" : "+v"(oo));
+                ap.add(nl_entry(jlo + (int)(oo >> 4), code));
+              }
             }
-          }
+          };
+          if (dx == 0 && dy == 0 && dz == 0)
+            scan((uint32_t)(p - jlo) * 16u, std::true_type{});
+          else
+            scan(0u, std::false_type{});
         } else {
           for (int32_t j = jb; j < je; ++j) cand(j, pos[j]);
         }
