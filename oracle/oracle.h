@@ -44,6 +44,11 @@ int orc_globals(void* h, double out[4]);
 /* Neighbour list of particle i in the current order (strict r < r_s test
  * on the stored representation); returns the count, writes up to cap. */
 int64_t orc_neighbors(void* h, int64_t i, int64_t* out, int64_t cap);
+/* Every particle's neighbour list at once: counts[i]; id0 and stencil cell
+ * code ((dx+1) << 4 | (dy+1) << 2 | (dz+1)) of each neighbour, in particle
+ * and walk order.  Returns the total entries; writes ids / codes only when
+ * cap holds them. */
+int64_t orc_neighbor_csr(void* h, int64_t* counts, int32_t* ids, int8_t* codes, int64_t cap);
 int64_t orc_escaped(void* h);
 /* Dynamic rigid object (shape matching) with its rest centre of mass. */
 int orc_set_rigid_object(void* h, int obj, const double cm[3]);

@@ -64,6 +64,14 @@ def build(force=False):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "liboracle.so"], check=True)
 
 
+def use_library(path):
+    """Load the oracle from ``path`` (e.g. the host-native CPU-baseline
+    build) instead of liboracle.so; takes effect for later calls."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -82,6 +90,9 @@ def lib():
         L.orc_neighbors.argtypes = [ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
         L.orc_neighbors.restype = ctypes.c_int64
+        L.orc_neighbor_csr.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_int64]
+        L.orc_neighbor_csr.restype = ctypes.c_int64
         L.orc_escaped.argtypes = [ctypes.c_void_p]
         L.orc_escaped.restype = ctypes.c_int64
         L.orc_set_threads.argtypes = [ctypes.c_int]
@@ -154,6 +165,16 @@ class Oracle:
         buf = (ctypes.c_int64 * cap)()
         k = lib().orc_neighbors(self.h, int(i), buf, cap)
         return np.array(buf[:min(k, cap)], dtype=np.int64)
+
+    def neighbor_csr(self):
+        """(counts, id0 of each neighbour, stencil codes) of every particle,
+        concatenated in particle and walk order."""
+        counts = np.zeros(self.n, dtype=np.int64)
+        total = lib().orc_neighbor_csr(self.h, counts.ctypes.data, None, None, 0)
+        ids = np.zeros(max(total, 1), dtype=np.int32)
+        codes = np.zeros(max(total, 1), dtype=np.int8)
+        lib().orc_neighbor_csr(self.h, counts.ctypes.data, ids.ctypes.data, codes.ctypes.data, total)
+        return counts, ids[:total], codes[:total]
 
     def escaped(self):
         return lib().orc_escaped(self.h)

@@ -9,7 +9,9 @@ sums in arbitrary order, full 3x3 tensors.  Agreement of the two to ~1e-10
 therefore pins the C++ oracle's grid, sort, pass order and per-pair
 formulas against a second reading of the reference:
 
-  step order ............ eng/solver_sph_base.py:41-51, 96-120, 228-247
+  step order ............ eng/solver_sph_base.py:41-51, 79-120, 228-247
+  RK4 ................... eng/solver_sph_base.py:134-180
+  collision boundary .... eng/solver_sph_base.py:524-601
   Shepard ............... eng/solver_sph_base.py:386-398
   Wendland / cubic ...... eng/solver_sph_base.py:298-358
   WC one_step ........... eng/solver_sph_wc.py:33-71, 82-133
@@ -419,6 +421,66 @@ class NumpySPH:
             self.seq[soil] += self.dt * self.dseq[soil]
             self.seqp[soil] += self.dt * self.dseqp[soil]
 
+    def rk4(self):
+        """substep_RK (solver_sph_base.py:134-180): four one_step calls
+        weighted 1, 2, 2, 1; each of the first three intermediate states is
+        the step-start state plus half a step of the last rate (advect_RK_4
+        uses 0.5 dt before every stage, the fourth included)."""
+        real = self.t > 0
+        soil = self.t == SOIL
+        acc_rho = np.zeros(self.n)
+        acc_v = np.zeros((self.n, 3))
+        acc_sig = np.zeros((self.n, 3, 3))
+        for k, m in enumerate((1, 2, 2, 1)):
+            self.one_step()
+            acc_rho[real] += self.drho[real] * m
+            acc_v[real] += self.dv[real] * m
+            if self.d["method"] == 3:
+                acc_sig[soil] += self.dsig[soil] * m
+            if k < 3:
+                self.rhot[real] = 0.5 * self.dt * self.drho[real] + self.rho[real]
+                self.mV[real] = self.mass[real] / self.rhot[real]
+                self.vt[real] = 0.5 * self.dt * self.dv[real] + self.v[real]
+                if self.d["method"] == 3:
+                    self.sigt[soil] = 0.5 * self.dt * self.dsig[soil] + self.sig[soil]
+        self.rho[real] += self.dt / 6.0 * acc_rho[real]
+        self.mV[real] = self.mass[real] / self.rho[real]
+        self.v[real] += self.dt / 6.0 * acc_v[real]
+        if self.d["method"] == 3:
+            self.sig[soil] += self.dt / 6.0 * acc_sig[soil]
+
+    def enforce_boundary(self):
+        """enforce_boundary_2D/3D + simulate_collisions
+        (solver_sph_base.py:524-601) for the particles judge_enforce_bdy
+        picks: flow particles with boundary 1, dynamic rigid ones always.
+        Per axis the position is clamped to the wall minus (radius - 1e-8)
+        (no top wall), the wall normals summed, and the velocity loses
+        1.3 times its component along the normalised sum."""
+        sel = ((self.d["boundary"] == 1) & is_flow(self.t)) | self.rig_dyn
+        if not sel.any():
+            return
+        ds, de = np.asarray(self.d["domain_start"]), np.asarray(self.d["domain_end"])
+        dist = self.d["particle_radius"] - 1e-8
+        pos = self.x[sel]  # the tests read the position before any clamp
+        x = pos.copy()
+        nrm = np.zeros_like(x)
+        axes = (0, 1, 2) if self.dim == 3 else (0, 1)
+        for a in axes:
+            if a != 1:  # the top wall is commented out in the reference
+                hi = pos[:, a] > de[a] - dist
+                nrm[hi, a] += 1.0
+                x[hi, a] = de[a] - dist
+            lo = pos[:, a] <= ds[a] + dist
+            nrm[lo, a] -= 1.0
+            x[lo, a] = ds[a] + dist
+        ln = np.sqrt(np.sum(nrm * nrm, axis=1))
+        hit = ln > 1e-8
+        u = np.where(hit[:, None], nrm / np.where(hit, ln, 1.0)[:, None], 0.0)
+        v = self.v[sel]
+        v = v - (1.3 * np.sum(v * u, axis=1))[:, None] * u
+        self.x[sel] = x
+        self.v[sel] = v
+
     def step(self):
         self.shepard()
         self.init_tmp()
@@ -426,6 +488,8 @@ class NumpySPH:
         if ti == 1:
             self.one_step()
             self.lf_full()
+        elif ti == 4:
+            self.rk4()
         else:
             self.one_step()
             self.lf_half()
@@ -434,6 +498,7 @@ class NumpySPH:
         self.advect_pos()
         self.advect_something()
         self.solve_rigid()
+        self.enforce_boundary()
 
     def solve_rigid(self):
         """solve_constraints (solver_sph_base.py:479-500): cm, A, the polar

@@ -1305,6 +1305,32 @@ struct Sim {
     });
     return k;
   }
+  // Every particle's neighbours (for_all_neighbors order) in one call:
+  // counts[i], then id0 of each neighbour and its stencil cell code
+  // (dx+1) << 4 | (dy+1) << 2 | (dz+1) (d = neighbour cell - own cell),
+  // concatenated in particle order.  Returns the total; fills ids / codes
+  // only when they hold it.
+  int64_t neighbor_csr(int64_t* counts, int32_t* ids, int8_t* codes, int64_t cap) const {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t k = 0;
+      walk(i, [&](int64_t, R, R, R, R) { ++k; });
+      counts[i] = k;
+    }
+    std::vector<int64_t> off(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + counts[i];
+    if (off[n] > cap || !ids || !codes) return off[n];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t k = off[i];
+      walk(i, [&](int64_t j, R, R, R, R) {
+        ids[k] = id0[j];
+        codes[k] = (int8_t)(((cx[j] - cx[i] + 1) << 4) | ((cy[j] - cy[i] + 1) << 2) | (cz[j] - cz[i] + 1));
+        ++k;
+      });
+    }
+    return off[n];
+  }
 };
 
 struct Handle {
@@ -1348,6 +1374,9 @@ int orc_set_rigid_object(void* h, int obj, const double cm[3]) { DISPATCH(h, S.a
 int orc_polar(const double* A, int dim, double* R) { return polar_rotation(A, dim, R), 0; }
 int orc_globals(void* h, double out[4]) { DISPATCH(h, (S.globals(out), 0)); }
 int64_t orc_neighbors(void* h, int64_t i, int64_t* out, int64_t cap) { DISPATCH(h, S.neighbors(i, out, cap)); }
+int64_t orc_neighbor_csr(void* h, int64_t* counts, int32_t* ids, int8_t* codes, int64_t cap) {
+  DISPATCH(h, S.neighbor_csr(counts, ids, codes, cap));
+}
 int64_t orc_escaped(void* h) { DISPATCH(h, S.escaped); }
 // ---- function-level hooks for closed-form tests -------------------------
 int orc_kernel_eval(const orc_params* p, int precision, const double* r, int64_t n, double* W,

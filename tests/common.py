@@ -100,3 +100,80 @@ def parity_err(a, b, floor):
     if b.size == 0:
         return 0.0
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+FLUID_T, SOIL_T, RIGID_T = 1, 2, 11
+
+
+def _list_keep(ti, tj):
+    """Which for_all_neighbors pairs a recorded list keeps: every neighbour
+    of a flow particle, flow neighbours of a boundary particle, flow and
+    rigid neighbours of a rigid one."""
+    flow_i = (ti == FLUID_T) | (ti == SOIL_T)
+    flow_j = (tj == FLUID_T) | (tj == SOIL_T)
+    bdy_i = ti < 0
+    return flow_i | (bdy_i & flow_j) | ((ti == RIGID_T) & (flow_j | (tj == RIGID_T)))
+
+
+def compare_neighbor_lists(eng, slot, o32, o64=None, fp64_stride=1, log=print):
+    """The engine's recorded neighbour lists against the oracle's neighbour
+    walk on the same (sorted) state:
+
+    * float32 oracle: for every particle the same neighbours in the same
+      order (ids and stencil codes, bit-exact), after the type filter of the
+      lists; a particle whose list overflowed (count -1) must have more
+      neighbours than any list holds;
+    * FP64 oracle (optional, every ``fp64_stride``-th particle): the
+      neighbour sets may differ only in knife-edge pairs, |x_ij| within 4
+      float32 ulps of r_s; their count is returned and logged.
+
+    Returns (entries compared, knife-edge pairs, differing pairs vs FP64)."""
+    cg, ids_g, codes_g = eng.neighbor_lists(slot)
+    cnt, ids_o, codes_o = o32.neighbor_csr()
+    n = cnt.size
+    assert cg.size == n
+    typ = o32.get("type")
+    id0 = o32.get("id0")
+    type_by_id0 = np.empty(n, dtype=typ.dtype)
+    type_by_id0[id0] = typ
+    owner = np.repeat(np.arange(n), cnt)
+    keep = _list_keep(typ[owner], type_by_id0[ids_o])
+    kcnt = np.bincount(owner[keep], minlength=n)
+    over = cg < 0
+    cap = 160 if o32.desc["dim"] == 3 else 48  # entries a list of the engine holds
+    assert np.all(kcnt[over] > cap), "a list overflowed below its capacity"
+    np.testing.assert_array_equal(cg[~over], kcnt[~over], err_msg="neighbour counts")
+    sel = keep & ~over[owner]
+    np.testing.assert_array_equal(ids_g, ids_o[sel], err_msg="neighbour ids / order")
+    np.testing.assert_array_equal(codes_g, codes_o[sel], err_msg="stencil codes")
+    knife = diff = 0
+    if o64 is not None:
+        c64, ids64, _ = o64.neighbor_csr()
+        rs = o32.desc["support_radius"]
+        ulp4 = 4.0 * float(np.spacing(np.float32(rs)))
+        pos = o64.get("pos")
+        pos_by_id0 = np.empty_like(pos)
+        pos_by_id0[o64.get("id0")] = pos
+        assert np.array_equal(o64.get("id0"), id0)
+        pick = np.zeros(n, dtype=bool)
+        pick[::fp64_stride] = True
+        own32 = owner
+        own64 = np.repeat(np.arange(n), c64)
+        a = own32[pick[own32]].astype(np.int64) * n + ids_o[pick[own32]]
+        b = own64[pick[own64]].astype(np.int64) * n + ids64[pick[own64]]
+        x = np.setxor1d(a, b, assume_unique=True)
+        diff = x.size
+        both = np.union1d(a, b)
+        i_own, j_id = both // n, both % n
+        r = np.linalg.norm(pos[i_own] - pos_by_id0[j_id], axis=1)
+        edge = np.abs(r - rs) <= ulp4
+        knife = int(edge.sum())
+        if diff:
+            xi, xj = x // n, x % n
+            rx = np.linalg.norm(pos[xi] - pos_by_id0[xj], axis=1)
+            assert np.all(np.abs(rx - rs) <= ulp4), \
+                f"{int(np.sum(np.abs(rx - rs) > ulp4))} pairs differ from FP64 away from r_s"
+    log(f"neighbour lists: {ids_g.size} entries bit-exact vs float32 oracle, {int(over.sum())} "
+        f"overflowed; vs FP64: {diff} differing pairs, all among {knife} knife-edge pairs "
+        f"(|r - r_s| <= 4 ulp)")
+    return ids_g.size, knife, diff
