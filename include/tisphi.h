@@ -294,6 +294,21 @@ int tsph_timing(void* engine, int enable);
  * the sum and sum of squares of their TSPH_F_ID0 values. */
 int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
                         int64_t* id_sqsum);
+/* The neighbour lists the pair passes replay (recorded by the last list
+ * build: TSPH_PASS_SHEPARD or a step, list mode 1 or 2), per particle of the
+ * slot in the current order: the pairs for_all_neighbors
+ * (particle_system.py:259-269) visits, in its order, of which each pass
+ * reads -- every neighbour of a flow particle, the flow neighbours of a
+ * boundary particle, the flow and rigid neighbours of a rigid one.
+ * count[i] = entries of particle i, or -1 when it has more neighbours than a
+ * list holds (its passes walk the cell stencil); ids / codes = each entry's
+ * TSPH_F_ID0 and stencil cell ((dx+1) << 4 | (dy+1) << 2 | (dz+1), d = the
+ * neighbour's cell minus i's), concatenated in particle and list order
+ * (room for ids_cap entries); *out_total = entries in all lists.
+ * TSPH_ERR_INVALID_ARG when no lists match the current order (list mode 0,
+ * or no build since the last sort or upload) or ids_cap is too small. */
+int tsph_neighbor_lists(void* engine, int slot, int32_t* count, int32_t* ids, int8_t* codes,
+                        int64_t ids_cap, int64_t* out_total);
 
 /* ---- single-scene convenience (slot 0 of a one-slot engine) ------------
  * The reference's one-scene ParticleSystem + solver pair

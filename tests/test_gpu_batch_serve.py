@@ -105,8 +105,11 @@ def test_escaped_particle_is_reported_not_faulted():
     e.close()
 
 
-def test_nan_is_reported():
-    sc = build_scene(scene_cfg("dam_break_2d"))
+@pytest.mark.parametrize("ti", [1, 2, 4])
+def test_nan_is_reported(ti):
+    cfg = scene_cfg("dam_break_2d")
+    cfg.config["Configuration"]["timeIntegration"] = ti
+    sc = build_scene(cfg)
     e = L.Engine(sc.method, sc.geo.dim, sc.n, sc.geo.grid_num_total, 1)
     s = e.add_scene(sc.desc())
     L.upload_scene(e, s, sc)

@@ -123,7 +123,8 @@ def test_capacity_limits_are_errors():
 def test_out_of_domain_inputs_are_argument_errors():
     """A position outside the cell grid over the virtual domain (the
     reference's pos_to_index would address a cell that does not exist) or
-    a cell id out of range is refused at upload; the engine stays usable."""
+    a cell id or an id0 out of range is refused at upload; the engine stays
+    usable."""
     sc = build_scene(tiny_cfg())
     eng = L.Engine(sc.method, 2, sc.n, sc.geo.grid_num_total, 1)
     slot = eng.add_scene(sc.desc())
@@ -141,6 +142,15 @@ def test_out_of_domain_inputs_are_argument_errors():
     cell[3] = sc.geo.grid_num_total
     with pytest.raises(ValueError):
         eng.upload(slot, "cell", cell)
+    # id0 indexes the scene's initial positions and colours on the device:
+    # a value outside [0, n) is refused before any kernel reads it
+    good = eng.download(slot, "id0")
+    for v in (-1, sc.n):
+        ids = good.copy()
+        ids[11] = v
+        with pytest.raises(ValueError):
+            eng.upload(slot, "id0", ids)
+        np.testing.assert_array_equal(eng.download(slot, "id0"), good)
     L.upload_scene(eng, slot, sc)
     eng.step(3)
     eng.sync()

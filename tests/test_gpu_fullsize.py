@@ -5,13 +5,18 @@
     order and cell ids bit-exact, every field within the parity tolerance;
   * the three neighbour modes (stencil walk in every pass, lists built from
     global memory, lists built from LDS windows) give bitwise the same state;
-  * the FP64 mass sum is unchanged by the steps.
+  * the FP64 mass sum is unchanged by the steps;
+  * the neighbour lists the LDS-staged build records hold, for every one of
+    the 4.5 M particles, exactly the float32 oracle's neighbours in its walk
+    order (ids and stencil codes); against the FP64 oracle (every 7th
+    particle) only knife-edge pairs at |x_ij| = r_s differ, and their number
+    is printed.
 """
 import numpy as np
 import pytest
 
 from oracle.oracle import Oracle, load_scene
-from tests.common import TOL, parity_err, scene_cfg
+from tests.common import TOL, compare_neighbor_lists, parity_err, scene_cfg
 from tisphi_amd import _lib as L
 from tisphi_amd.scene import build_scene
 
@@ -94,4 +99,26 @@ def test_full_size_long_run_is_stable(scene):
     assert g[1] < g0[1] and g[2] > 0.0
     # released energy is bounded by the potential energy lost
     assert g[2] < g0[1] - g[1]
+    eng.close()
+
+
+def test_full_size_neighbor_lists_exact(scene):
+    sc = scene
+    eng = L.Engine(sc.method, 3, sc.n, sc.geo.grid_num_total, 1)
+    slot = eng.add_scene(sc.desc())
+    L.upload_scene(eng, slot, sc)
+    eng.set_neighbor_list(2)
+    eng.run("sort")
+    eng.run("shepard")
+    o32 = Oracle(sc.desc(), 32)
+    load_scene(o32, sc)
+    o32.run("sort")
+    np.testing.assert_array_equal(eng.download(slot, "id0"), o32.get("id0"))
+    o64 = Oracle(sc.desc(), 64)
+    for f in ("cell", "offset", "vel", "density", "mass", "m_V", "type", "id0"):
+        o64.set(f, o32.get(f))
+    o64.run("sort")
+    nent, knife, diff = compare_neighbor_lists(eng, slot, o32, o64, fp64_stride=7,
+                                               log=lambda m: print("full size: " + m, flush=True))
+    assert nent > 250_000_000
     eng.close()

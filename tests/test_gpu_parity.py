@@ -4,7 +4,9 @@ Each case runs the float32 oracle for a few hundred steps to reach a
 non-trivial state, uploads that exact state (int cell + float32 offset and
 every field) to the engine, and then compares pass by pass:
   * sort: cell ids, particle order and cell ranges bit-exact;
-  * neighbour sets: per-particle count / id sums identical;
+  * neighbour lists: per particle the float32 oracle's neighbours in its
+    order (ids and stencil codes bit-exact, both list builds); against the
+    FP64 oracle only knife-edge pairs (|x_ij| within 4 ulps of r_s) differ;
   * every later pass: |gpu - oracle| <= 1e-5 * max(|oracle|, floor), with the
     floors rho0 (densities), rho0*|g|*H (pressure, stress), |g| (d_vel) and
     max|oracle| for rates.
@@ -18,8 +20,8 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle, load_scene
-from tests.common import (TOL, add_obj_bodies, add_rigid_blocks, box_mesh, parity_err, scene_cfg,
-                          write_obj)
+from tests.common import (TOL, add_obj_bodies, add_rigid_blocks, box_mesh, compare_neighbor_lists,
+                          parity_err, scene_cfg, write_obj)
 from tisphi_amd import _lib as L
 from tisphi_amd.scene import build_scene
 
@@ -193,19 +195,25 @@ def test_sort_bit_exact_and_neighbor_sets(case):
     for f in ("cell", "id0", "type", "cell_start"):
         np.testing.assert_array_equal(eng.download(slot, f), o2.get(f), err_msg=f)
     np.testing.assert_array_equal(eng.download(slot, "offset"), o2.get("offset"))
-    # neighbour sets: count and id0 sums per particle
-    c, s1, s2 = eng.neighbor_stats(slot)
-    id0 = o2.get("id0")
-    oc = np.zeros(sc.n, np.int64)
-    o1 = np.zeros(sc.n, np.int64)
-    o2s = np.zeros(sc.n, np.int64)
-    for i in range(sc.n):
-        nb = id0[o2.neighbors(i)].astype(np.int64)
-        oc[i], o1[i], o2s[i] = len(nb), nb.sum(), (nb * nb).sum()
-    mism = np.nonzero((c != oc) | (s1 != o1) | (s2 != o2s))[0]
-    # Only knife-edge pairs (|x_ij| == r_s on the lattice to the last ulp) may
-    # be decided differently; there must be none or very few.
-    assert len(mism) <= max(2, sc.n // 10000), f"{len(mism)} particles with different neighbour sets"
+    # neighbour lists: the LDS-staged build (mode 2) and the global build
+    # (mode 1) record exactly the float32 oracle's neighbours in its order;
+    # against FP64 only knife-edge pairs may differ
+    o64 = Oracle(sc.desc(), 64)
+    for f in state_fields(sc.method, sc):
+        o64.set(f, o2.get(f))
+    o64.run("sort")
+    prev = None
+    for mode in (2, 1):
+        eng.set_neighbor_list(mode)
+        eng.run("shepard")
+        nent, knife, diff = compare_neighbor_lists(eng, slot, o2, o64 if mode == 2 else None,
+                                                   log=lambda m: print(f"[{key} mode {mode}] {m}"))
+        lists = eng.neighbor_lists(slot)
+        if prev is not None:
+            for a, b in zip(prev, lists):
+                np.testing.assert_array_equal(a, b)
+        prev = lists
+    assert nent > 0
     eng.close()
 
 

@@ -33,7 +33,8 @@ EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_eng
            "tsph_engine_destroy", "tsph_engine_bytes", "tsph_add_scene", "tsph_remove_scene",
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
            "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list", "tsph_globals",
-           "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats", "tsph_create",
+           "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats",
+           "tsph_neighbor_lists", "tsph_create",
            "tsph_destroy", "tsph_assign_value_color", "tsph_v_maxmin", "tsph_set_color",
            "tsph_set_rigid_object"]
 
@@ -117,6 +118,7 @@ def load():
         "tsph_kernel_time": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(i64)], i32),
         "tsph_timing": ([vp, i32], i32),
         "tsph_neighbor_stats": ([vp, i32, vp, vp, vp], i32),
+        "tsph_neighbor_lists": ([vp, i32, vp, vp, vp, i64, ctypes.POINTER(i64)], i32),
         "tsph_create": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(vp)], i32),
         "tsph_destroy": ([vp], i32),
         "tsph_assign_value_color": ([vp, i32, i32, i32, ctypes.c_uint32], i32),
@@ -257,6 +259,23 @@ class Engine:
         check(load().tsph_neighbor_stats(self.h, int(slot), c.ctypes.data, s1.ctypes.data,
                                          s2.ctypes.data))
         return c, s1, s2
+
+    def neighbor_lists(self, slot=0):
+        """(count, ids, codes) of the recorded neighbour lists of ``slot``:
+        count[i] entries of particle i (-1: it walks the stencil), the
+        neighbours' id0 and stencil codes concatenated in list order."""
+        n = self.n[slot]
+        c = np.zeros(n, dtype=np.int32)
+        total = ctypes.c_int64(0)
+        st = load().tsph_neighbor_lists(self.h, int(slot), c.ctypes.data, None, None, 0,
+                                        ctypes.byref(total))
+        if st not in (0, -1):
+            check(st)
+        ids = np.zeros(max(total.value, 1), dtype=np.int32)
+        codes = np.zeros(max(total.value, 1), dtype=np.int8)
+        check(load().tsph_neighbor_lists(self.h, int(slot), c.ctypes.data, ids.ctypes.data,
+                                         codes.ctypes.data, ids.size, ctypes.byref(total)))
+        return c, ids[:total.value], codes[:total.value]
 
     def set_rigid_object(self, slot, obj, rest_cm):
         """Register a dynamic rigid object (isDynamic block) of ``slot``."""

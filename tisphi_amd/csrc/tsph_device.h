@@ -412,10 +412,10 @@ __device__ __forceinline__ uint32_t nl_entry(int j, int code) {
 // entries k..k+3 with one 16-byte load.  (Runs of 4, which make the
 // replay's loads one contiguous kilobyte per wave, measured ~4 % slower per
 // step.)  nl_cap is a multiple of 16.
-__device__ __forceinline__ size_t nl_base(int p, int nl_cap) {
+__host__ __device__ __forceinline__ size_t nl_base(int p, int nl_cap) {
   return (size_t)(p >> 6) * ((size_t)nl_cap << 6) + ((size_t)(p & 63) << 4);
 }
-__device__ __forceinline__ size_t nl_at(int k) {
+__host__ __device__ __forceinline__ size_t nl_at(int k) {
   return ((size_t)(k >> 4) << 10) + (size_t)(k & 15);
 }
 

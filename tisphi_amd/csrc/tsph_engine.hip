@@ -893,7 +893,15 @@ int transfer(tsph_engine* E, int slot, int field, void* host, size_t bytes, bool
       for (int64_t i = 0; i < n; ++i) hi[i] = v[i];
       return TSPH_OK;
     }
-    case TSPH_F_ID0: return i1(cur.id0);
+    case TSPH_F_ID0:
+      // kernels index the scene's id0-ordered arrays (initial positions and
+      // colours) with it: a value outside [0, n) would read past the scene
+      if (up)
+        for (int64_t i = 0; i < n; ++i)
+          if (hi[i] < 0 || hi[i] >= n)
+            return fail(TSPH_ERR_INVALID_ARG, "id0 " + std::to_string(hi[i]) + " out of range [0, " +
+                                                  std::to_string(n) + ")");
+      return i1(cur.id0);
     case TSPH_F_OBJ_ID: return i1(cur.obj);
     case TSPH_F_CELL:
       if (up)
@@ -1075,34 +1083,50 @@ int tsph_add_scene(void* engine, const tsph_scene_desc* desc, int* out_slot) {
   SceneDev S = make_scene(E, desc);
   S.pbase = (int32_t)pb;
   S.cbase = (int32_t)cb;
-  HIP_TRY(hipStreamSynchronize(E->stream));
-  // Static particle -> slot map and zeroed state for the range.
-  std::vector<int32_t> sid(desc->n_particles, slot);
-  if ((r = h2d(E, E->w.sid, pb, sid))) return r;
-  for (int b = 0; b < 2; ++b) {
-    PState& P = E->buf[b];
-    const size_t n = desc->n_particles;
-    if (n == 0) break;
-    HIP_TRY(hipMemsetAsync(P.cell + pb, 0, n * 4, E->stream));
-    HIP_TRY(hipMemsetAsync(P.pos + pb, 0, n * sizeof(float4), E->stream));
-    HIP_TRY(hipMemsetAsync(P.vel + pb, 0, n * sizeof(float4), E->stream));
-    HIP_TRY(hipMemsetAsync(P.vtmp + pb, 0, n * sizeof(float4), E->stream));
-    HIP_TRY(hipMemsetAsync(P.mass + pb, 0, n * 4, E->stream));
-    HIP_TRY(hipMemsetAsync(P.type + pb, 0, n, E->stream));
-    for (int c = 0; c < 6; ++c) {
-      if (P.sig_f) HIP_TRY(hipMemsetAsync(P.sig_f + c * (size_t)E->cap + pb, 0, n * 4, E->stream));
-      if (P.sig_d) HIP_TRY(hipMemsetAsync(P.sig_d + c * (size_t)E->cap + pb, 0, n * 8, E->stream));
+  // Static particle -> slot map and zeroed state for the range; on any
+  // failure the reserved ranges go back to the allocators.
+  r = [&]() -> int {
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    std::vector<int32_t> sid(desc->n_particles, slot);
+    int rr;
+    if ((rr = h2d(E, E->w.sid, pb, sid))) return rr;
+    for (int b = 0; b < 2; ++b) {
+      PState& P = E->buf[b];
+      const size_t n = desc->n_particles;
+      if (n == 0) break;
+      HIP_TRY(hipMemsetAsync(P.cell + pb, 0, n * 4, E->stream));
+      HIP_TRY(hipMemsetAsync(P.pos + pb, 0, n * sizeof(float4), E->stream));
+      HIP_TRY(hipMemsetAsync(P.vel + pb, 0, n * sizeof(float4), E->stream));
+      HIP_TRY(hipMemsetAsync(P.vtmp + pb, 0, n * sizeof(float4), E->stream));
+      HIP_TRY(hipMemsetAsync(P.mass + pb, 0, n * 4, E->stream));
+      HIP_TRY(hipMemsetAsync(P.type + pb, 0, n, E->stream));
+      for (int c = 0; c < 6; ++c) {
+        if (P.sig_f) HIP_TRY(hipMemsetAsync(P.sig_f + c * (size_t)E->cap + pb, 0, n * 4, E->stream));
+        if (P.sig_d) HIP_TRY(hipMemsetAsync(P.sig_d + c * (size_t)E->cap + pb, 0, n * 8, E->stream));
+      }
+      if (P.seq) HIP_TRY(hipMemsetAsync(P.seq + pb, 0, n * 8, E->stream));
+      if (P.seqp) HIP_TRY(hipMemsetAsync(P.seqp + pb, 0, n * 8, E->stream));
+      if (P.flag) HIP_TRY(hipMemsetAsync(P.flag + pb, 0, n * 4, E->stream));
     }
-    if (P.seq) HIP_TRY(hipMemsetAsync(P.seq + pb, 0, n * 8, E->stream));
-    if (P.seqp) HIP_TRY(hipMemsetAsync(P.seqp + pb, 0, n * 8, E->stream));
-    if (P.flag) HIP_TRY(hipMemsetAsync(P.flag + pb, 0, n * 4, E->stream));
+    HIP_TRY(hipMemsetAsync(E->w.status + 2 * slot, 0, 16, E->stream));
+    return TSPH_OK;
+  }();
+  if (r) {
+    E->palloc.release(pb);
+    E->calloc_.release(cb);
+    return r;
   }
-  HIP_TRY(hipMemsetAsync(E->w.status + 2 * slot, 0, 16, E->stream));
   E->scenes[slot] = S;
   E->descs[slot] = *desc;
   E->nl_valid = E->cflow_valid = E->fresh_sort = false;
   E->steps[slot] = 0;
-  if ((r = upload_tables(E))) return r;
+  if ((r = upload_tables(E))) {
+    const std::string msg = g_err;
+    E->scenes[slot].active = 0;
+    E->palloc.release(pb);
+    E->calloc_.release(cb);
+    return fail(r, msg);
+  }
   *out_slot = slot;
   return TSPH_OK;
 }
@@ -1382,6 +1406,51 @@ int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
   if (d1) hipFree(d1);
   if (d2) hipFree(d2);
   return r;
+}
+
+int tsph_neighbor_lists(void* engine, int slot, int32_t* count, int32_t* ids, int8_t* codes,
+                        int64_t ids_cap, int64_t* out_total) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!slot_ok(E, slot) || !count || !out_total || ids_cap < 0 || (ids_cap > 0 && (!ids || !codes)))
+    return fail(TSPH_ERR_INVALID_ARG, "bad argument");
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  if (!E->use_nl || !E->nl_valid)
+    return fail(TSPH_ERR_INVALID_ARG, "no neighbour lists match the current particle order "
+                                      "(list mode 0, or no list build since the last sort / upload)");
+  const SceneDev& S = E->scenes[slot];
+  const int64_t n = S.n, base = S.pbase;
+  if (base % 64 != 0) return fail(TSPH_ERR_INVALID_ARG, "scene range not tile aligned");
+  const int64_t ntiles = (n + 63) / 64, per_tile = (int64_t)E->nl_cap * 64;
+  std::vector<int32_t> nlc, id0;
+  std::vector<uint32_t> nl;
+  int r;
+  if ((r = d2h(E, (const int32_t*)E->w.nlc, base, n, nlc)) ||
+      (r = d2h(E, (const int32_t*)E->buf[E->parity].id0, base, n, id0)) ||
+      (r = d2h(E, (const uint32_t*)E->w.nl, (base / 64) * per_tile, ntiles * per_tile, nl)))
+    return r;
+  int64_t total = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t c = nlc[i];
+    if (c < 0 || c > E->nl_cap) {
+      count[i] = -1;  // walks the stencil in the pair passes
+      continue;
+    }
+    count[i] = c;
+    const int64_t lb = (i >> 6) * per_tile + ((i & 63) << 4);
+    for (int32_t k = 0; k < c; ++k) {
+      const uint32_t e = nl[lb + (int64_t)nl_at(k)];
+      const int64_t j = (int64_t)(e >> kNlShift) - base;
+      if (j < 0 || j >= n) return fail(TSPH_ERR_HIP, "neighbour list entry outside the scene");
+      if (total < ids_cap) {
+        ids[total] = id0[j];
+        codes[total] = (int8_t)(e & 63u);
+      }
+      ++total;
+    }
+  }
+  *out_total = total;
+  if (total > ids_cap) return fail(TSPH_ERR_INVALID_ARG, "ids_cap too small: " + std::to_string(total) + " entries");
+  return TSPH_OK;
 }
 
 int tsph_create(const tsph_scene_desc* desc, void** out_engine) {

@@ -1694,6 +1694,8 @@ __global__ __launch_bounds__(256) void k_rk_advect(KArgs a) {
     v.y += S.dt6 * d.y;
     v.z += S.dt6 * d.z;
     a.cur.vel[p] = v;
+    // the NaN status of k_lf_full, for the RK4 update of the same state
+    if (isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w)) atomicAdd(&a.w.status[2 * s + 1], 1ull);
   }
   if (t == kSoil && S.method == kDP) {
     const int st = a.stride;
