@@ -294,6 +294,14 @@ int tsph_timing(void* engine, int enable);
  * the sum and sum of squares of their TSPH_F_ID0 values. */
 int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
                         int64_t* id_sqsum);
+/* tsph_neighbor_stats without any allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops).  Same outputs for one slot of
+ * at most 262,144 particles (TSPH_ERR_INVALID_ARG beyond); its scratch is a
+ * fixed 5 MiB device array of the library that every GPU holds once, shared
+ * by all calls on that GPU: calls that overlap on one GPU race. */
+int tsph_neighbor_stats_noalloc(void* engine, int slot, int32_t* count, int64_t* id_sum,
+                                int64_t* id_sqsum);
 /* The neighbour lists the pair passes replay (recorded by the last list
  * build: TSPH_PASS_SHEPARD or a step, list mode 1 or 2), per particle of the
  * slot in the current order: the pairs for_all_neighbors

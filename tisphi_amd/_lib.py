@@ -34,7 +34,7 @@ EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_eng
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
            "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list", "tsph_globals",
            "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats",
-           "tsph_neighbor_lists", "tsph_create",
+           "tsph_neighbor_lists", "tsph_neighbor_stats_noalloc", "tsph_create",
            "tsph_destroy", "tsph_assign_value_color", "tsph_v_maxmin", "tsph_set_color",
            "tsph_set_rigid_object"]
 
@@ -119,6 +119,7 @@ def load():
         "tsph_timing": ([vp, i32], i32),
         "tsph_neighbor_stats": ([vp, i32, vp, vp, vp], i32),
         "tsph_neighbor_lists": ([vp, i32, vp, vp, vp, i64, ctypes.POINTER(i64)], i32),
+        "tsph_neighbor_stats_noalloc": ([vp, i32, vp, vp, vp], i32),
         "tsph_create": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(vp)], i32),
         "tsph_destroy": ([vp], i32),
         "tsph_assign_value_color": ([vp, i32, i32, i32, ctypes.c_uint32], i32),
@@ -258,6 +259,17 @@ class Engine:
         s2 = np.zeros(n, dtype=np.int64)
         check(load().tsph_neighbor_stats(self.h, int(slot), c.ctypes.data, s1.ctypes.data,
                                          s2.ctypes.data))
+        return c, s1, s2
+
+    def neighbor_stats_noalloc(self, slot=0):
+        """neighbor_stats through the entry point that allocates nothing
+        (a slot of at most 262,144 particles)."""
+        n = self.n[slot]
+        c = np.zeros(n, dtype=np.int32)
+        s1 = np.zeros(n, dtype=np.int64)
+        s2 = np.zeros(n, dtype=np.int64)
+        check(load().tsph_neighbor_stats_noalloc(self.h, int(slot), c.ctypes.data, s1.ctypes.data,
+                                                 s2.ctypes.data))
         return c, s1, s2
 
     def neighbor_lists(self, slot=0):

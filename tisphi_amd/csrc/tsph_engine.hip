@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -77,6 +78,42 @@ struct RangeAlloc {
   }
   void release(int64_t s) { used.erase(s); }
 };
+
+// Scratch of tsph_neighbor_stats_noalloc: the per-particle count, id sum and
+// id square sum of one slot of at most kStatsScratchMax particles, in a
+// file-scope device array (every GPU the library runs on holds its own copy
+// until the process ends), so that entry point allocates nothing.  Host
+// code uses the array only through the device address
+// hipGetSymbolAddress returns on each device (kept per device below), never
+// through its name.
+constexpr int64_t kStatsScratchMax = 262144;
+constexpr size_t kStatsCountBytes = (size_t)kStatsScratchMax * 4;
+constexpr size_t kStatsScratchBytes = kStatsCountBytes + 2 * (size_t)kStatsScratchMax * 8;
+__device__ __align__(256) char scratch[kStatsScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller's current device is restored).
+int scratch_on(int device, char** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(TSPH_ERR_INVALID_ARG, "device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int prev = -1;
+    HIP_TRY(hipGetDevice(&prev));
+    if (prev != device) HIP_TRY(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != device) HIP_TRY(hipSetDevice(prev));
+    if (e != hipSuccess || !ptr)
+      return fail(TSPH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<char*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return TSPH_OK;
+}
 
 struct Timing {
   bool on = false;
@@ -1004,6 +1041,14 @@ int tsph_engine_create(int method, int dim, int64_t particle_capacity, int64_t c
   E->dim = dim;
   E->max_scenes = max_scenes;
   hipGetDevice(&E->device);
+  {
+    // set-up of tsph_neighbor_stats_noalloc's scratch on this device
+    char* scr = nullptr;
+    if (int rs = scratch_on(E->device, &scr)) {
+      delete E;
+      return rs;
+    }
+  }
   E->cap = (int32_t)pc;
   E->ccap = (int32_t)cell_capacity;
   // 3D lattice: 92 neighbours (+30 exactly at r_s); 2D: 24 (+4).  Denser
@@ -1390,9 +1435,9 @@ int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
   if (!r) {
     Launcher L{E};
     if (E->dim == 2)
-      L.go("neighbor_stats", k_neighbor_stats<2>, E->nblocks_tiles(256), 256, E->args(), dc, d1, d2);
+      L.go("neighbor_stats", k_neighbor_stats<2>, E->nblocks_tiles(256), 256, E->args(), dc, d1, d2, -1, 0);
     else
-      L.go("neighbor_stats", k_neighbor_stats<3>, E->nblocks_tiles(256), 256, E->args(), dc, d1, d2);
+      L.go("neighbor_stats", k_neighbor_stats<3>, E->nblocks_tiles(256), 256, E->args(), dc, d1, d2, -1, 0);
     r = L.status;
   }
   if (!r) {
@@ -1406,6 +1451,39 @@ int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
   if (d1) hipFree(d1);
   if (d2) hipFree(d2);
   return r;
+}
+
+int tsph_neighbor_stats_noalloc(void* engine, int slot, int32_t* count, int64_t* id_sum,
+                                int64_t* id_sqsum) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!slot_ok(E, slot) || !count || !id_sum || !id_sqsum) return fail(TSPH_ERR_INVALID_ARG, "bad argument");
+  const SceneDev& S = E->scenes[slot];
+  if (S.n > kStatsScratchMax)
+    return fail(TSPH_ERR_INVALID_ARG, "neighbor_stats_noalloc: " + std::to_string(S.n) +
+                                          " particles in the slot, at most " +
+                                          std::to_string(kStatsScratchMax));
+  char* base = nullptr;
+  int r = scratch_on(E->device, &base);
+  if (r) return r;
+  int32_t* dc = reinterpret_cast<int32_t*>(base);
+  long long* d1 = reinterpret_cast<long long*>(base + kStatsCountBytes);
+  long long* d2 = d1 + kStatsScratchMax;
+  // Launched directly (no timing events are created).  Every particle of
+  // the slot writes its three entries, so nothing needs clearing; the array
+  // keeps the last call's contents, and calls that overlap on one GPU race.
+  if (E->dim == 2)
+    hipLaunchKernelGGL(k_neighbor_stats<2>, dim3(E->nblocks_tiles(256)), dim3(256), 0, E->stream, E->args(), dc,
+                       d1, d2, slot, S.pbase);
+  else
+    hipLaunchKernelGGL(k_neighbor_stats<3>, dim3(E->nblocks_tiles(256)), dim3(256), 0, E->stream, E->args(), dc,
+                       d1, d2, slot, S.pbase);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+  if (e == hipSuccess && S.n > 0) e = hipMemcpy(count, dc, S.n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && S.n > 0) e = hipMemcpy(id_sum, d1, S.n * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && S.n > 0) e = hipMemcpy(id_sqsum, d2, S.n * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(TSPH_ERR_HIP, std::string("neighbor_stats_noalloc: ") + hipGetErrorString(e));
+  return TSPH_OK;
 }
 
 int tsph_neighbor_lists(void* engine, int slot, int32_t* count, int32_t* ids, int8_t* codes,

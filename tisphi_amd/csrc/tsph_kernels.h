@@ -1706,12 +1706,16 @@ __global__ __launch_bounds__(256) void k_rk_advect(KArgs a) {
 
 // Neighbour-set digest for tests: count, sum and sum of squares of the
 // neighbours' id0 (for_all_neighbors membership, particle_system.py:268).
+// only_slot < 0: every particle, output at its engine index; else only the
+// particles of that slot, output at (index - base).
 template <int DIM>
 __global__ __launch_bounds__(256) void k_neighbor_stats(KArgs a, int32_t* __restrict__ cnt,
                                                         long long* __restrict__ s1,
-                                                        long long* __restrict__ s2) {
+                                                        long long* __restrict__ s2, int only_slot,
+                                                        int32_t base) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
+  if (only_slot >= 0 && s != only_slot) return;
   const SceneDev S = a.scenes[s];
   const Centre C = centre_of(a, S, p);
   const int32_t* __restrict__ id0 = a.cur.id0;
@@ -1723,9 +1727,10 @@ __global__ __launch_bounds__(256) void k_neighbor_stats(KArgs a, int32_t* __rest
     a1 += id;
     a2 += id * id;
   });
-  cnt[p] = n;
-  s1[p] = a1;
-  s2[p] = a2;
+  const int o = p - base;
+  cnt[o] = n;
+  s1[o] = a1;
+  s2[o] = a2;
 }
 
 // =================================================================== globals
