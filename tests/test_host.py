@@ -40,6 +40,50 @@ def test_vtu_round_trip(tmp_path):
         np.testing.assert_array_equal(d[k], v)
 
 
+def test_vtu_appended_layout(tmp_path):
+    """The file follows the VTK XML layout pyevtk's pointsToVTK writes:
+    every DataArray 'appended' at an offset into one raw AppendedData
+    section, each block = UInt64 byte count + little-endian data, blocks in
+    header order (points interleaved x, y, z; Int32 connectivity and
+    offsets; UInt8 VTK_VERTEX types; the dump arrays)."""
+    import xml.etree.ElementTree as ET
+    pos, data = _dump(n=37, seed=2)
+    p = export.export_vtu(str(tmp_path / "c"), pos, data)
+    raw = open(p, "rb").read()
+    k = raw.index(b'<AppendedData encoding="raw">\n_')
+    base = k + len(b'<AppendedData encoding="raw">\n_')
+    assert raw.endswith(b"\n</AppendedData>\n</VTKFile>")
+    root = ET.fromstring(raw[:k] + b"<AppendedData/></VTKFile>")
+    assert root.attrib == {"type": "UnstructuredGrid", "version": "1.0", "byte_order": "LittleEndian",
+                           "header_type": "UInt64"}
+    piece = root.find("UnstructuredGrid/Piece")
+    assert piece.attrib == {"NumberOfPoints": "37", "NumberOfCells": "37"}
+    das = piece.findall(".//DataArray")
+    names = [d.get("Name") for d in das]
+    assert names == ["points", "connectivity", "offsets", "types"] + list(data)
+    assert piece.find("PointData").get("Scalars") == "id0"
+    size = {"Float64": 8, "Int64": 8, "Int32": 4, "UInt8": 1}
+    dt = {"Float64": "<f8", "Int64": "<i8", "Int32": "<i4", "UInt8": "u1"}
+    off = 0
+    blocks = {}
+    for d in das:
+        assert d.get("format") == "appended" and int(d.get("offset")) == off
+        nb = int(np.frombuffer(raw, "<u8", 1, base + off)[0])
+        ncomp = int(d.get("NumberOfComponents"))
+        assert nb == 37 * ncomp * size[d.get("type")]
+        blocks[d.get("Name")] = np.frombuffer(raw, dt[d.get("type")], 37 * ncomp, base + off + 8)
+        off += 8 + nb
+    assert base + off == len(raw) - len(b"\n</AppendedData>\n</VTKFile>")
+    pts = blocks["points"].reshape(37, 3)
+    for a, key in enumerate(("pos.x", "pos.y", "pos.z")):
+        np.testing.assert_array_equal(pts[:, a], pos[key])
+    np.testing.assert_array_equal(blocks["connectivity"], np.arange(37))
+    np.testing.assert_array_equal(blocks["offsets"], np.arange(1, 38))
+    np.testing.assert_array_equal(blocks["types"], np.ones(37))
+    for key, v in data.items():
+        np.testing.assert_array_equal(blocks[key], v)
+
+
 def test_stamps():
     assert export.step_stamp(42) == "000042"
     assert export.time_stamp(0.34922) == "time.secx1e6.0349220"
