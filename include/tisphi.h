@@ -294,6 +294,15 @@ int tsph_timing(void* engine, int enable);
  * the sum and sum of squares of their TSPH_F_ID0 values. */
 int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
                         int64_t* id_sqsum);
+/* Self-check of the pair distance the pair kernels use (v_sqrt_f32 plus a
+ * one-ulp residual correction, in place of the correctly rounded sqrtf and
+ * its small-argument rescaling): over every finite non-negative float,
+ * out[0] = results that differ from sqrtf at arguments >= 2^-96, out[1] =
+ * results outside [0, 1e-8) below 2^-96 (where the smoothing kernels are 0
+ * for either).  Both 0 means every kernel value equals the one formed
+ * from the correctly rounded root, as the reference's |x_ij| (.norm()).
+ * Runs on the current device. */
+int tsph_check_sqrt(uint64_t out[2]);
 /* tsph_neighbor_stats without any allocation (no counterpart in the
  * reference; for callers that cannot allocate device memory: code under
  * HIP graph capture, tight serving loops).  Same outputs for one slot of

@@ -156,3 +156,12 @@ def test_out_of_domain_inputs_are_argument_errors():
     eng.sync()
     assert eng.status(slot) == (0, 0)
     eng.close()
+
+
+def test_pair_distance_root_equals_sqrtf():
+    """The pair kernels take |x_ij| from v_sqrt_f32 with a one-ulp residual
+    correction instead of the correctly rounded sqrtf: over every finite
+    non-negative float it must give sqrtf's bits at arguments >= 2^-96 and
+    a value below 1e-8 (a zero kernel value either way) under it."""
+    bad, small = L.check_sqrt()
+    assert (bad, small) == (0, 0)

@@ -323,21 +323,21 @@ def test_neighbor_list_path_equals_stencil_walk(case):
 
 @pytest.mark.parametrize("name", ["column_collapse_mui", "column_collapse_dp"])
 def test_granular_globals_vs_fp64(name):
-    """The 2D granular column collapse (80,000 soil particles, dt 1e-5):
-    300 fused f32 steps (DP return mapping in FP64) against the FP64
-    oracle.  Measured: front and potential energy within 1e-7, kinetic
-    energy within 1e-4 (muI) / 3e-6 (DP), mass within 1e-7; asserted at 1 %
-    (front, energies) and 1e-6 (mass)."""
+    """The 2D granular column collapse (80,000 soil particles, dt 1e-5) for
+    its full 2,000 steps: fused f32 steps (DP return mapping in FP64)
+    against the FP64 oracle; front, energies within 1 %, mass within 1e-6
+    (the measured errors are printed)."""
+    steps = 2000
     sc = build_scene(scene_cfg(name))
     assert sc.n_flow == 80_000
     eng, slot = new_engine(sc)
     L.upload_scene(eng, slot, sc)
-    eng.step(300)
+    eng.step(steps)
     eng.sync()
     g = eng.globals(slot)
     o64 = Oracle(sc.desc(), 64)
     load_scene(o64, sc)
-    o64.step(300)
+    o64.step(steps)
     g64 = o64.globals()
     rel = np.abs(np.asarray(g) - g64) / np.abs(g64)
     print(f"{name}: gpu {g} fp64 {g64} rel {rel}")

@@ -34,9 +34,17 @@ EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_eng
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
            "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list", "tsph_globals",
            "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats",
-           "tsph_neighbor_lists", "tsph_neighbor_stats_noalloc", "tsph_create",
+           "tsph_neighbor_lists", "tsph_neighbor_stats_noalloc", "tsph_check_sqrt", "tsph_create",
            "tsph_destroy", "tsph_assign_value_color", "tsph_v_maxmin", "tsph_set_color",
            "tsph_set_rigid_object"]
+
+
+def check_sqrt():
+    """(differences from sqrtf at arguments >= 2^-96, out-of-range results
+    below) of the pair-distance root over every finite non-negative float."""
+    out = (ctypes.c_uint64 * 2)()
+    check(load().tsph_check_sqrt(out))
+    return int(out[0]), int(out[1])
 
 
 class SceneDesc(ctypes.Structure):
@@ -120,6 +128,7 @@ def load():
         "tsph_neighbor_stats": ([vp, i32, vp, vp, vp], i32),
         "tsph_neighbor_lists": ([vp, i32, vp, vp, vp, i64, ctypes.POINTER(i64)], i32),
         "tsph_neighbor_stats_noalloc": ([vp, i32, vp, vp, vp], i32),
+        "tsph_check_sqrt": ([ctypes.POINTER(ctypes.c_uint64)], i32),
         "tsph_create": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(vp)], i32),
         "tsph_destroy": ([vp], i32),
         "tsph_assign_value_color": ([vp, i32, i32, i32, ctypes.c_uint32], i32),

@@ -138,6 +138,26 @@ struct KArgs {
   int32_t nl_cap;  // neighbour-list capacity per particle
 };
 
+// ---------------------------------------------------------- pair distance
+// |x_ij| from r2 >= 0: v_sqrt_f32 (within an ulp), then the neighbouring
+// float whose square is on the right side of r2 (the residual sign decides,
+// with exact fused multiply-adds).  Equal bit for bit to the correctly
+// rounded sqrtf for every r2 >= 2^-96 (checked over all those floats by
+// tsph_check_sqrt); below 2^-96 it returns a value under 1e-8, which the
+// kernels below map to 0 exactly as they map sqrtf's (r > 1e-8 test).  It
+// drops sqrtf's rescaling of tiny arguments and its special-value fixup:
+// 7 of its 16 instructions per pair.
+__device__ __forceinline__ float pair_dist(float r2) {
+  const float s = __builtin_amdgcn_sqrtf(r2);
+  const float sd = __int_as_float(__float_as_int(s) - 1);
+  const float su = __int_as_float(__float_as_int(s) + 1);
+  const float rd = fmaf(-sd, s, r2);
+  const float ru = fmaf(-su, s, r2);
+  float r = rd <= 0.0f ? sd : s;
+  r = ru > 0.0f ? su : r;
+  return r;
+}
+
 // ------------------------------------------------------------- kernels W
 // Wendland C2 (solver_sph_base.py:334-358) and cubic spline (:298-330).
 __device__ __forceinline__ float kernel_W(const SceneDev& S, float rn) {

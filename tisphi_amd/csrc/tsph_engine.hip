@@ -132,10 +132,10 @@ struct Timing {
 // captured graph is valid only for the options it was captured with).
 struct StepFlags {
   int ti = 2;
-  bool xsph = false, collide = false, kcorr = false, rigid = false;
+  bool xsph = false, collide = false, kcorr = false, rigid = false, rep = false;
   bool operator==(const StepFlags& o) const {
     return ti == o.ti && xsph == o.xsph && collide == o.collide && kcorr == o.kcorr &&
-           rigid == o.rigid;
+           rigid == o.rigid && rep == o.rep;
   }
 };
 
@@ -536,6 +536,7 @@ StepFlags step_flags(tsph_engine* E) {
     f.collide = f.collide || E->scenes[s].boundary == 1;
     f.kcorr = f.kcorr || E->scenes[s].kcorr != 0;
     f.rigid = f.rigid || E->scenes[s].nrig > 0;
+    f.rep = f.rep || E->scenes[s].rep != 0;
   }
   return f;
 }
@@ -1451,6 +1452,26 @@ int tsph_neighbor_stats(void* engine, int slot, int32_t* count, int64_t* id_sum,
   if (d1) hipFree(d1);
   if (d2) hipFree(d2);
   return r;
+}
+
+int tsph_check_sqrt(uint64_t out[2]) {
+  if (!out) return fail(TSPH_ERR_INVALID_ARG, "null argument");
+  unsigned long long* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, 2 * sizeof(unsigned long long)));
+  hipError_t e = hipMemset(d, 0, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) {
+    // every finite non-negative float: [+0, +inf)
+    hipLaunchKernelGGL(k_check_pair_dist, dim3(8192), dim3(256), 0, nullptr, 0u, 0x7F800000u, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  unsigned long long h[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return fail(TSPH_ERR_HIP, std::string("check_sqrt: ") + hipGetErrorString(e));
+  out[0] = h[0];
+  out[1] = h[1];
+  return TSPH_OK;
 }
 
 int tsph_neighbor_stats_noalloc(void* engine, int slot, int32_t* count, int64_t* id_sum,

@@ -620,7 +620,7 @@ __global__ __launch_bounds__(256) void k_shepard_list(KArgs a, int skip_fluid) {
   float sum = 0.0f;
   neighbours_k<DIM, true>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
-    if (is_flow(type[j])) sum += mVj * kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
+    if (is_flow(type[j])) sum += mVj * kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
   });
   a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
 }
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(256) void k_cspm_L(KArgs a) {
     neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                           float mVj, int) {
       if (type[j] != ti) return;
-      const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
+      const float f = kernel_gradfac_t<decltype(kt)::v>(S, pair_dist(r2));
       const float g[3] = {f * x, f * y, f * z};
       const float c[3] = {mVj * (-x), mVj * (-y), mVj * (-z)};
 #pragma unroll
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(256) void k_shepard(KArgs a, int skip_fluid) {
   const int8_t* __restrict__ type = a.cur.type;
   float sum = 0.0f;
   walk_c<DIM>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
-    if (is_flow(type[j])) sum += mVj * kernel_W(S, sqrtf(r2));
+    if (is_flow(type[j])) sum += mVj * kernel_W(S, pair_dist(r2));
   });
   a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
 }
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
     if (!is_flow(type[j])) return;
-    const float w = kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
+    const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
     const float4 vj = vtmp[j];
     svx += mVj * vj.x * w;
     svy += mVj * vj.y * w;
@@ -757,6 +757,42 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
 // solver_sph_base.py:386-398: flow neighbours, list order), which the fused
 // step then leaves out of the Shepard pass: in WC only boundary particles
 // read CSPM_f within the step, so fluid particles get it here, bitwise equal.
+// The pair terms of one fluid particle, one neighbour at a time in list
+// order: d_density (solver_sph_base.py:195-203), viscosity and pressure
+// force (solver_sph_wc.py:33-71) and, with SHEP, the Shepard sum.  q is the
+// neighbour's pair record (v_tmp, p / rho_tmp^2, sign bit set for a wall).
+template <int KT, bool SHEP>
+struct WcPairSum {
+  float vix, viy, viz, pri, c1, c2, rho0, rr;
+  float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f, shep = 0.f;
+  __device__ __forceinline__ void add(const SceneDev& S, float x, float y, float z, float r2, float mVj,
+                                      const float4& q) {
+    const float rn = pair_dist(r2);
+    const float f = kernel_gradfac_t<KT>(S, rn);
+    const float gx = f * x, gy = f * y, gz = f * z;
+    const bool bj = __float_as_int(q.w) < 0;  // boundary neighbour
+    const float ux = vix - q.x, uy = viy - q.y, uz = viz - q.z;
+    dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
+    if (SHEP) shep += bj ? 0.0f : mVj * kernel_W_t<KT>(S, rn);
+    // viscosity_force: fluid and boundary neighbours (solver_sph_wc.py:33-45)
+    const float vx_ = ux * x + uy * y + uz * z;
+    const float mn = vx_ < 0.0f ? vx_ : 0.0f;
+    const float a0 = c1 * mVj;
+    const float aa = bj ? a0 * rr : a0;
+    const float vis = aa * mn / (r2 + c2);
+    const float pc = (-rho0 * mVj) * (pri + fabsf(q.w));
+    const float sc = vis + pc;
+    dvx += sc * gx;
+    dvy += sc * gy;
+    dvz += sc * gz;
+  }
+  __device__ __forceinline__ void store(const KArgs& a, const SceneDev& S, int p, float rho_i) const {
+    a.w.drho[p] = dd * rho_i;
+    a.w.dv[p] = make_float4(dvx + S.grav[0], dvy + S.grav[1], dvz + S.grav[2], 0.0f);
+    if (SHEP) a.w.cspm[p] = shep != 0.0f ? 1.0f / shep : 1.0f;
+  }
+};
+
 template <int DIM, bool NL, bool SHEP, int KT, bool KC>
 __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, int p) {
   const Centre C = centre_of(a, S, p);
@@ -770,22 +806,40 @@ __device__ __forceinline__ void wc_fluid_one(const KArgs& a, const SceneDev& S, 
   const float pri = vp[p].w;
   const float c1 = S.visc_c1, c2 = S.visc_c2, rho0 = S.rho0;
   const float rr = rho0 / vi.w;  // rho0 / rho_i of the boundary viscosity
+  if (!KC) {
+    WcPairSum<KT, SHEP> acc{vi.x, vi.y, vi.z, pri, c1, c2, rho0, rr};
+    neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
+      acc.add(S, x, y, z, r2, mVj, vp[j]);
+    });
+    if (S.rep) {
+      // calc_rep_force_task (solver_sph_wc.py:74-76, 118-121): a second
+      // sweep adding the repulsive particles' force after the pair forces
+      const int8_t* __restrict__ type = a.cur.type;
+      neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float, int) {
+        if (type[j] != kRep) return;
+        const float f = rep_fac(S, sqrtf(r2));
+        acc.dvx += f * x;
+        acc.dvy += f * y;
+        acc.dvz += f * z;
+      });
+    }
+    acc.store(a, S, p, vi.w);
+    return;
+  }
   float M[9];
-  if (KC) corr_load(a.w.Lc, a.stride, p, M);
+  corr_load(a.w.Lc, a.stride, p, M);
   float dd = 0.f, dvx = 0.f, dvy = 0.f, dvz = 0.f, shep = 0.f;
   neighbours<DIM, NL>(S, a, p, C, [&](int j, float x, float y, float z, float r2, float mVj, int) {
-    const float rn = sqrtf(r2);
+    const float rn = pair_dist(r2);
     const float f = kernel_gradfac_t<KT>(S, rn);
     const float gx = f * x, gy = f * y, gz = f * z;
     const float4 q = vp[j];
     const bool bj = __float_as_int(q.w) < 0;  // boundary neighbour
     const float ux = vi.x - q.x, uy = vi.y - q.y, uz = vi.z - q.z;
-    if (KC) {  // d_density takes the corrected gradient, the forces do not
+    {  // d_density takes the corrected gradient, the forces do not
       float cg[3] = {gx, gy, gz};
       corr_apply(M, cg);
       dd += (mVj * ux) * cg[0] + (mVj * uy) * cg[1] + (mVj * uz) * cg[2];
-    } else {
-      dd += (mVj * ux) * gx + (mVj * uy) * gy + (mVj * uz) * gz;
     }
     if (SHEP) shep += bj ? 0.0f : mVj * kernel_W_t<KT>(S, rn);
     // viscosity_force: fluid and boundary neighbours (solver_sph_wc.py:33-45)
@@ -837,7 +891,7 @@ __device__ __forceinline__ float4 wc_rigid_force(const KArgs& a, const SceneDev&
   neighbours<DIM, NL>(S, a, p, C, [&](int i, float x, float y, float z, float r2, float, int) {
     if (type[i] != kFluid) return;
     const float xi = -x, yi = -y, zi = -z;  // x_i - x_j
-    const float rn = sqrtf(r2);
+    const float rn = pair_dist(r2);
     const float f = kernel_gradfac_t<KT>(S, rn);
     const float gx = f * xi, gy = f * yi, gz = f * zi;
     const float4 vi = vtmp[i];
@@ -908,7 +962,7 @@ __global__ __launch_bounds__(256) void k_mui_soil1(KArgs a) {
   float dd = 0.f;
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
-    const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
+    const float f = kernel_gradfac_t<decltype(kt)::v>(S, pair_dist(r2));
     float gw[3] = {f * x, f * y, f * z};
     if (kc) corr_apply(M, gw);
     const float4 vj = vtmp[j];
@@ -970,7 +1024,7 @@ __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
     if (!is_flow(type[j])) return;
-    const float w = kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
+    const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
     const float4 vj = vtmp[j];
     sv[0] += mVj * vj.x * w;
     sv[1] += mVj * vj.y * w;
@@ -1039,7 +1093,7 @@ __device__ __forceinline__ float4 soil_rigid_force(const KArgs& a, const SceneDe
   float dv[3] = {S.grav[0], S.grav[1], S.grav[2]};
   neighbours<DIM, NL>(S, a, p, C, [&](int i, float x, float y, float z, float r2, float, int) {
     if (type[i] != kSoil) return;
-    const float f = kernel_gradfac_t<KT>(S, sqrtf(r2));
+    const float f = kernel_gradfac_t<KT>(S, pair_dist(r2));
     float gw[3] = {f * -x, f * -y, f * -z};
     if (kc) {
       float M[9];
@@ -1100,7 +1154,7 @@ __global__ __launch_bounds__(256) void k_mui_soil3(KArgs a) {
   if (kc) corr_load(a.w.Lc, a.stride, p, M);
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
-    const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
+    const float f = kernel_gradfac_t<decltype(kt)::v>(S, pair_dist(r2));
     float gw[3] = {f * x, f * y, f * z};
     if (kc) corr_apply(M, gw);
     soil_dv_pair<DIM>(S, sgt, st, j, si, mVj, vtmp[j].w, gw, dv);
@@ -1160,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_mui_regu(KArgs a) {
   walk<DIM>(S, a.w.start, a.cur.pos, C.shift, p, sx, sy, sz, C.cx, C.cy, C.cz, C.P.x, C.P.y,
             C.P.z, [&](int j, float x, float y, float z, float r2, float mVj, int) {
               if (type[j] != ti) return;
-              const float w = kernel_W(S, sqrtf(r2));
+              const float w = kernel_W(S, pair_dist(r2));
 #pragma unroll
               for (int c = 0; c < 6; ++c) acc[c] += mVj * sg[c * st + j] * w;
             });
@@ -1344,7 +1398,7 @@ __global__ __launch_bounds__(256) void k_dp_soil(KArgs a) {
   if (kc) corr_load(a.w.Lc, a.stride, p, M);
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
-    const float f = kernel_gradfac_t<decltype(kt)::v>(S, sqrtf(r2));
+    const float f = kernel_gradfac_t<decltype(kt)::v>(S, pair_dist(r2));
     float gw[3] = {f * x, f * y, f * z};
     if (kc) corr_apply(M, gw);
     const float4 vj = vtmp[j];
@@ -1536,7 +1590,7 @@ __global__ __launch_bounds__(256) void k_xsph(KArgs a) {
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
     if (type[j] != ti) return;
-    const float w = kernel_W_t<decltype(kt)::v>(S, sqrtf(r2));
+    const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
     const float4 vj = vel[j];
     sx += mVj * (vj.x - vi.x) * w;
     sy += mVj * (vj.y - vi.y) * w;
@@ -1731,6 +1785,24 @@ __global__ __launch_bounds__(256) void k_neighbor_stats(KArgs a, int32_t* __rest
   cnt[o] = n;
   s1[o] = a1;
   s2[o] = a2;
+}
+
+// Exhaustive check of pair_dist against the correctly rounded sqrtf over
+// the float bit patterns [lo, hi): out[0] counts differences at arguments
+// >= 2^-96 (bit pattern 0x0F800000), out[1] results outside [0, 1e-8)
+// below it.  Grid-stride over a fixed grid; no memory is read.
+__global__ __launch_bounds__(256) void k_check_pair_dist(uint32_t lo, uint32_t hi,
+                                                         unsigned long long* __restrict__ out) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  unsigned long long bad = 0, small = 0;
+  for (uint64_t i = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += stride) {
+    const float x = __uint_as_float((uint32_t)i);
+    const float r = pair_dist(x);
+    if (i >= 0x0F800000u) bad += __float_as_uint(r) != __float_as_uint(sqrtf(x));
+    else small += !(r >= 0.0f && r < 1e-8f);
+  }
+  if (bad) atomicAdd(&out[0], bad);
+  if (small) atomicAdd(&out[1], small);
 }
 
 // =================================================================== globals
