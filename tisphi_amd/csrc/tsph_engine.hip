@@ -372,7 +372,11 @@ struct Seq {
   }
   // calc_CSPM_f; with neighbour lists on, the walk also records the lists.
   // skip_fluid: fluid particles get theirs in the step's first fluid pass.
-  void shepard(bool skip_fluid = false) {
+  // skip_all: the WC fused step, whose first fluid and boundary passes form
+  // every particle's Shepard sum themselves (a WC scene holds fluid,
+  // boundary and rigid particles only; the boundary pass covers the latter
+  // two).
+  void shepard(bool skip_fluid = false, bool skip_all = false) {
     if (E->use_nl) {
       const int cf = E->cflow_valid ? 1 : 0, cull = E->fresh_sort ? 1 : 0;
       if (E->nl_lds)
@@ -380,8 +384,9 @@ struct Seq {
       else
         L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args(), cf, cull);
       E->nl_valid = true;
-      L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
-    } else {
+      if (!skip_all)
+        L.go("shepard", k_shepard_list<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
+    } else if (!skip_all) {
       L.go("shepard", k_shepard<DIM>, pb(), E->pair_block, E->args(), skip_fluid ? 1 : 0);
     }
     // calc_kernel_corr (solver_sph_base.py:363-368): CSPM_L after CSPM_f
@@ -404,7 +409,8 @@ struct Seq {
     const int B = E->pair_block;
     if (E->method == kWC) {
       if (with_eos) L.go("wc_eos", k_wc_eos, tb(), E->elem_block, a);
-      L.go("wc_bdy", k_wc_bdy<DIM, NL>, pb(), B, a);
+      if (shep) L.go("wc_bdy", k_wc_bdy<DIM, NL, true>, pb(), B, a);
+      else L.go("wc_bdy", k_wc_bdy<DIM, NL, false>, pb(), B, a);
       if (shep) L.go("wc_fluid", k_wc_fluid<DIM, NL, true>, pb(), B, a);
       else L.go("wc_fluid", k_wc_fluid<DIM, NL, false>, pb(), B, a);
     } else if (E->method == kMUI) {
@@ -491,8 +497,10 @@ struct Seq {
   // the engine's time-integration scheme.
   void step(int ti, bool xsph, bool collide, bool rigid) {
     sort(true);
-    const bool shep = E->method == kWC;  // fluid Shepard sums in the fluid pass
-    shepard(shep);
+    // WC: every particle's Shepard sum is formed in the first boundary and
+    // fluid passes (the boundary pass runs first and reads it right away)
+    const bool shep = E->method == kWC;
+    shepard(shep, shep);
     if (ti == 2) {
       one_step(false, shep);
       lf_half(true);

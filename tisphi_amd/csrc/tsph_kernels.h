@@ -716,7 +716,10 @@ __global__ __launch_bounds__(256) void k_wc_eos(KArgs a) {  // solver_sph_wc.py:
 }
 
 // Dummy extrapolation (solver_sph_wc.py:90-103; solver_sph_base.py:653-663).
-template <int DIM, bool NL>
+// With SHEP the same walk first forms the particle's Shepard sum
+// (calc_CSPM_f, solver_sph_base.py:386-398: flow neighbours, list order),
+// which the fused step then leaves out of the Shepard pass: bitwise equal.
+template <int DIM, bool NL, bool SHEP>
 __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   int s, p;
   if (!tile_thread(a, s, p)) return;
@@ -727,18 +730,25 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float* __restrict__ pr = a.w.p;
   const float gy = S.grav[1];
-  float svx = 0.f, svy = 0.f, svz = 0.f, sp = 0.f;
+  float svx = 0.f, svy = 0.f, svz = 0.f, sp = 0.f, shep = 0.f;
   neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
     if (!is_flow(type[j])) return;
     const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
+    if (SHEP) shep += mVj * w;
     const float4 vj = vtmp[j];
     svx += mVj * vj.x * w;
     svy += mVj * vj.y * w;
     svz += mVj * vj.z * w;
     sp += mVj * (pr[j] + vj.w * gy * y) * w;
   });
-  const float cs = a.w.cspm[p];
+  float cs;
+  if (SHEP) {
+    cs = shep != 0.0f ? 1.0f / shep : 1.0f;
+    a.w.cspm[p] = cs;
+  } else {
+    cs = a.w.cspm[p];
+  }
   const float4 v = a.cur.vel[p];
   const float4 vt = make_float4(2.0f * v.x - svx * cs, 2.0f * v.y - svy * cs,
                                 2.0f * v.z - svz * cs, S.rho0);
