@@ -62,9 +62,13 @@ struct SceneDev {
 // for a static rigid or any other particle.
 __device__ __forceinline__ int rigid_slot(const SceneDev& S, int type, int obj) {
   if (type != kRigid) return -1;
-  for (int k = 0; k < S.nrig; ++k)
-    if (S.rig_obj[k] == obj) return k;
-  return -1;
+  // constant indices only: a runtime index into the scene record would put
+  // the whole record on the stack of every kernel that calls this
+  int r = -1;
+#pragma unroll
+  for (int k = kMaxRigid - 1; k >= 0; --k)
+    if (k < S.nrig && S.rig_obj[k] == obj) r = k;
+  return r;
 }
 
 // Particle state that survives a step (double-buffered; the counting sort

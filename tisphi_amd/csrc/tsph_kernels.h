@@ -1944,7 +1944,10 @@ __global__ __launch_bounds__(256) void k_rigid_tile(KArgs a) {
     const size_t id = (size_t)S.pbase + (size_t)a.cur.id0[p];
     const int st = a.stride;
     double q[3];
-    for (int c = 0; c < 3; ++c) q[c] = a.w.x0[(size_t)c * st + id] - S.rig_cm0[k][c];
+    // the initial centre read from the scene table (a runtime index into
+    // the local copy S would put S on the stack)
+    const double* cm0 = a.scenes[t.x].rig_cm0[k];
+    for (int c = 0; c < 3; ++c) q[c] = a.w.x0[(size_t)c * st + id] - cm0[c];
     v[0] = m;
     for (int c = 0; c < 3; ++c) v[1 + c] = m * x[c];
     for (int r = 0; r < 3; ++r)
@@ -2047,17 +2050,23 @@ __global__ __launch_bounds__(256) void k_rigid_apply(KArgs a) {
   const size_t id = (size_t)S.pbase + (size_t)a.cur.id0[p];
   const int st = a.stride;
   double q[3], x[3];
-  for (int c = 0; c < 3; ++c) q[c] = a.w.x0[(size_t)c * st + id] - S.rig_cm0[k][c];
+  const double* cm0 = a.scenes[s].rig_cm0[k];  // as in k_rigid_tile
+  for (int c = 0; c < 3; ++c) q[c] = a.w.x0[(size_t)c * st + id] - cm0[c];
   float4 P = a.cur.pos[p];
   abs_position(S, a.cur.cell[p], P, x);
   int cc[3];
   decode_cell(S, a.cur.cell[p], cc[0], cc[1], cc[2]);
-  float* o[3] = {&P.x, &P.y, &P.z};
-  for (int r = 0; r < S.dim; ++r) {
+  float o[3] = {P.x, P.y, P.z};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {  // constant indices: no stack array
+    if (r >= S.dim) continue;
     const double goal = cr[r] + (cr[3 + 3 * r] * q[0] + cr[4 + 3 * r] * q[1] + cr[5 + 3 * r] * q[2]);
     const double xn = x[r] + (goal - x[r]) * 1.0;
-    *o[r] = (float)(xn - S.vds[r] - (double)cc[r] * S.grid);
+    o[r] = (float)(xn - S.vds[r] - (double)cc[r] * S.grid);
   }
+  P.x = o[0];
+  P.y = o[1];
+  P.z = o[2];
   a.cur.pos[p] = P;
 }
 
