@@ -226,8 +226,10 @@ def main():
     ap.add_argument("--nl", type=int, default=2, choices=(0, 1, 2),
                     help="neighbour search: 0 stencil walk per pass, 1 lists, 2 LDS-built lists (default)")
     ap.add_argument("--block", type=str, default="256,256", help="pair,elem threads per block")
+    ap.add_argument("--cell-tile", type=int, default=256, choices=(128, 256, 512),
+                    help="candidates per LDS-staged window of the list build")
     ap.add_argument("--tune", action="store_true",
-                    help="pick block sizes and list mode with the autotuner (untimed steps) instead")
+                    help="pick block sizes, list mode and cell tile with the autotuner (untimed steps) instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -273,9 +275,10 @@ def main():
         shape = tuned["shape"]
     else:
         pair, elem = (int(v) for v in args.block.split(","))
-        shape = (pair, elem, args.nl)
+        shape = (pair, elem, args.nl, args.cell_tile)
         eng.set_block(pair, elem)
         eng.set_neighbor_list(shape[2])
+        eng.set_cell_tile(shape[3])
     eng.step(args.warmup)
     eng.sync()
     say(f"{args.warmup} warm-up steps done")
@@ -345,7 +348,8 @@ def main():
                    "particles": sc.n, "fluid_particles": sc.n_flow,
                    "cells": sc.geo.grid_num_total, "parallelism": "single GPU",
                    "launch_shape": {"pair_block": shape[0], "elem_block": shape[1],
-                                    "neighbour_lists": ["walk", "global", "lds"][shape[2]]}},
+                                    "neighbour_lists": ["walk", "global", "lds"][shape[2]],
+                                    "cell_tile": shape[3]}},
         "roofline": roof,
         "step_roofline": step,
         "kernel_ms_per_step": kernels,

@@ -223,6 +223,12 @@ int tsph_set_block(void* engine, int pair_block, int elem_block);
  * global memory; 0 walks the cell stencil in every pass.  All visit the same pairs in the same
  * order (for_all_neighbors, eng/particle_system.py:259-269). */
 int tsph_set_neighbor_list(void* engine, int enable);
+/* Cell tile of the mode-2 list build: how many candidate particles one
+ * wave stages in LDS per row offset (128, 256 (default) or 512; 2, 4 or
+ * 8 KiB per wave).  A window that does not fit is walked from global
+ * memory instead, so every size records the same lists.  No reference
+ * counterpart (a launch shape, like tsph_set_block). */
+int tsph_set_cell_tile(void* engine, int stage);
 
 /* ---- reductions --------------------------------------------------------- */
 /* FP64 global quantities of one scene (no reference counterpart; the

@@ -25,7 +25,7 @@ def test_tuned_steps_equal_default_steps(tmp_path, monkeypatch):
     e1, s1 = _engine(sc)
     r = tune(e1, steps=1, quick=False)
     k = r["steps_taken"]
-    assert k == 18 * 2 and not r["cached"]
+    assert k == 36 * 2 and not r["cached"]
     e1.step(5)
     e1.sync()
     e2, s2 = _engine(sc)

@@ -10,7 +10,7 @@
     the 4.5 M particles, exactly the float32 oracle's neighbours in its walk
     order (ids and stencil codes); against the FP64 oracle (every 7th
     particle) only knife-edge pairs at |x_ij| = r_s differ, and their number
-    is printed.
+    is printed; cell tiles of 128 and 512 candidates record the same lists.
 """
 import numpy as np
 import pytest
@@ -121,4 +121,15 @@ def test_full_size_neighbor_lists_exact(scene):
     nent, knife, diff = compare_neighbor_lists(eng, slot, o32, o64, fp64_stride=7,
                                                log=lambda m: print("full size: " + m, flush=True))
     assert nent > 250_000_000
+    # every cell-tile size of the build records the same lists (128
+    # candidates do not hold every window here, so the global-memory walk
+    # of the build is exercised too)
+    ref = eng.neighbor_lists(slot)
+    for tile in (128, 512):
+        eng.set_cell_tile(tile)
+        eng.run("shepard")
+        got = eng.neighbor_lists(slot)
+        for g, r, what in zip(got, ref, ("counts", "ids", "codes")):
+            np.testing.assert_array_equal(g, r, err_msg=f"cell tile {tile}: {what}")
+        print(f"full size: cell tile {tile}: lists equal", flush=True)
     eng.close()

@@ -304,13 +304,15 @@ def test_1000_step_globals_vs_fp64():
 
 def test_neighbor_list_path_equals_stencil_walk(case):
     """Lists recorded in one stencil walk per step (from global memory, or
-    from LDS-staged cell windows) replay the same pairs in the same order as
-    walking the stencil in every pass: bitwise equal state."""
+    from LDS-staged cell windows of every cell-tile size) replay the same
+    pairs in the same order as walking the stencil in every pass: bitwise
+    equal state."""
     key, sc, orc, H = case
     res = []
-    for use_nl in (0, 1, 2):
+    for use_nl, tile in ((0, 256), (1, 256), (2, 256), (2, 128), (2, 512)):
         eng, slot = new_engine(sc)
         eng.set_neighbor_list(use_nl)
+        eng.set_cell_tile(tile)
         copy_state(orc, eng, slot, sc.method, sc)
         eng.step(3)
         eng.sync()

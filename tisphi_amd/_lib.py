@@ -32,7 +32,8 @@ PASSES = dict(sort=0, shepard=1, init_tmp=2, one_step=3, lf_half=4, lf_full=5,
 EXPORTS = ["tsph_abi_version", "tsph_last_error", "tsph_device_count", "tsph_engine_create",
            "tsph_engine_destroy", "tsph_engine_bytes", "tsph_add_scene", "tsph_remove_scene",
            "tsph_upload", "tsph_download", "tsph_step", "tsph_sync", "tsph_run_pass",
-           "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list", "tsph_globals",
+           "tsph_scene_steps", "tsph_set_graph", "tsph_set_block", "tsph_set_neighbor_list",
+           "tsph_set_cell_tile", "tsph_globals",
            "tsph_status", "tsph_kernel_time", "tsph_timing", "tsph_neighbor_stats",
            "tsph_neighbor_lists", "tsph_neighbor_stats_noalloc", "tsph_check_sqrt", "tsph_create",
            "tsph_destroy", "tsph_assign_value_color", "tsph_v_maxmin", "tsph_set_color",
@@ -121,6 +122,7 @@ def load():
         "tsph_set_graph": ([vp, i32], i32),
         "tsph_set_block": ([vp, i32, i32], i32),
         "tsph_set_neighbor_list": ([vp, i32], i32),
+        "tsph_set_cell_tile": ([vp, i32], i32),
         "tsph_globals": ([vp, i32, dp], i32),
         "tsph_status": ([vp, i32, ctypes.POINTER(i64)], i32),
         "tsph_kernel_time": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(i64)], i32),
@@ -247,6 +249,11 @@ class Engine:
 
     def set_block(self, pair, elem):
         check(load().tsph_set_block(self.h, int(pair), int(elem)))
+
+    def set_cell_tile(self, stage):
+        """Candidates one wave stages in LDS per row offset in the mode-2
+        list build: 128, 256 (default) or 512."""
+        check(load().tsph_set_cell_tile(self.h, int(stage)))
 
     def globals(self, slot=0):
         out = (ctypes.c_double * 4)()

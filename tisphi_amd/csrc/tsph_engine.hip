@@ -148,6 +148,7 @@ struct tsph_engine {
   bool use_graph = true;
   bool use_nl = true;    // build per-step neighbour lists
   bool nl_lds = true;    // build them from LDS-staged cell windows
+  int nl_stage = kStageDefault;  // candidates per staged window (cell tile)
   bool nl_valid = false; // lists match the current order and positions
   bool cflow_valid = false;  // per-cell flow flags match the current order
   bool fresh_sort = false;   // offsets renormalised, no motion since the sort
@@ -379,9 +380,15 @@ struct Seq {
   void shepard(bool skip_fluid = false, bool skip_all = false) {
     if (E->use_nl) {
       const int cf = E->cflow_valid ? 1 : 0, cull = E->fresh_sort ? 1 : 0;
-      if (E->nl_lds)
-        L.go("nl_build", k_nl_build_lds<DIM>, E->tiles_alloc, 64, E->args(), cf, cull);
-      else
+      if (E->nl_lds) {
+        const KArgs a = E->args();
+        if (E->nl_stage == 128)
+          L.go("nl_build", k_nl_build_lds<DIM, 128>, E->tiles_alloc, 64, a, cf, cull);
+        else if (E->nl_stage == 512)
+          L.go("nl_build", k_nl_build_lds<DIM, 512>, E->tiles_alloc, 64, a, cf, cull);
+        else
+          L.go("nl_build", k_nl_build_lds<DIM, 256>, E->tiles_alloc, 64, a, cf, cull);
+      } else
         L.go("nl_build", k_nl_build<DIM>, pb(), E->pair_block, E->args(), cf, cull);
       E->nl_valid = true;
       if (!skip_all)
@@ -1303,6 +1310,17 @@ int tsph_set_neighbor_list(void* engine, int enable) {
   E->use_nl = enable != 0;
   E->nl_lds = enable == 2;
   E->nl_valid = false;
+  invalidate_graph(E);
+  return TSPH_OK;
+}
+
+int tsph_set_cell_tile(void* engine, int stage) {
+  tsph_engine* E = (tsph_engine*)engine;
+  if (!E) return fail(TSPH_ERR_INVALID_ARG, "null engine");
+  if (stage != 128 && stage != 256 && stage != 512)
+    return fail(TSPH_ERR_INVALID_ARG, "cell tile must be 128, 256 or 512 candidates");
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  E->nl_stage = stage;
   invalidate_graph(E);
   return TSPH_OK;
 }

@@ -432,11 +432,14 @@ __global__ __launch_bounds__(256) void k_nl_build(KArgs a, int use_cflow, int cu
 // in 3D and y in 2D; a tile's particles usually share one row, so for each
 // row offset (dx, dy in 3D; dx in 2D) the neighbour cells of all its lanes
 // are consecutive keys and their particles one contiguous range.  The wave
-// copies that range (coalesced, at most kStage particles) into LDS and every
-// lane walks its own three fast-axis cells there, in the order of walk()
-// (same pairs, same order, same values).  A tile spanning two rows, or a
-// larger range, reads global memory instead.
-constexpr int kStage = 256;  // staged candidates per row offset (4 KiB)
+// copies that range (coalesced, at most STAGE particles: the cell tile) into
+// LDS and every lane walks its own three fast-axis cells there, in the order
+// of walk() (same pairs, same order, same values).  A tile spanning two
+// rows, or a larger range, reads global memory instead.  STAGE (128, 256 or
+// 512 candidates: 2, 4 or 8 KiB of LDS per wave) trades how often a window
+// fits against how many waves a CU holds; the engine's cell-tile setting
+// picks it.
+constexpr int kStageDefault = 256;
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -455,9 +458,9 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-template <int DIM>
+template <int DIM, int STAGE>
 __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int cull) {
-  __shared__ float4 sp[kStage];
+  __shared__ float4 sp[STAGE];
   const int w = __builtin_amdgcn_readfirstlane((int)xcd_block());
   const int2 t = a.tiles[w];
   if (t.x < 0) return;  // unused tile: the whole workgroup leaves
@@ -530,7 +533,7 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
       const int khi = wave_max(act ? rkey + fhi : INT32_MIN);
       const int32_t jlo = start[klo] + shift;
       const int32_t cnt = start[khi + 1] + shift - jlo;
-      const bool staged = cnt <= kStage;  // wave-uniform
+      const bool staged = cnt <= STAGE;  // wave-uniform
       if (staged) {
         __syncthreads();  // the previous row offset's reads are done
         // w = 2 for a flow particle (the only neighbours a boundary lane
