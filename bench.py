@@ -114,6 +114,27 @@ def granular(steps=500, check_steps=300):
     return out, samples
 
 
+STEP_KERNELS = ("hash", "scan", "scatter", "rank_move", "shepard", "nl_build", "init_tmp", "wc_eos", "wc_bdy",
+                "wc_fluid", "lf_half", "lf_full", "advect_pos", "chk_density", "finish")
+
+
+def kernel_breakdown(eng, steps):
+    """Microseconds per step of each kernel of the step, eager launches with
+    HIP events around each (the engine is left in graph mode)."""
+    eng.set_graph(False)
+    eng.timing(True)
+    eng.step(steps)
+    eng.sync()
+    out = {}
+    for k in STEP_KERNELS:
+        ms, n = eng.kernel_time(k)
+        if n:
+            out[k] = round(1e3 * ms * n / steps, 2)
+    eng.timing(False)
+    eng.set_graph(True)
+    return out
+
+
 def batched_vs_plain(n_scenes=256, steps=100, plain_steps=10, n_check=8):
     """256 uneven 2D dam breaks (seed 0): scene-steps/s advanced together in
     one engine (hipGraph replay) and one scene per engine, one after the
@@ -125,6 +146,9 @@ def batched_vs_plain(n_scenes=256, steps=100, plain_steps=10, n_check=8):
     t_b, b = run_batched(scenes, steps, graph=True)
     picks = list(range(0, n_scenes, n_scenes // n_check))[:n_check]
     samples = [(("batched", k), scenes[k], steps + 1, b.engine.globals(b.slots[k])) for k in picks]
+    # where the batched step's time goes: its kernels one by one (the sum
+    # against the graph step says what launch boundaries cost)
+    kern = kernel_breakdown(b.engine, 20)
     b.close()
     say(f"batched {n_scenes} scenes x {steps} steps in {t_b:.3f} s")
     best = None
@@ -138,7 +162,9 @@ def batched_vs_plain(n_scenes=256, steps=100, plain_steps=10, n_check=8):
     batched = n_scenes * steps / t_b
     return dict(scenes=n_scenes, particles=sum(s.n for s in scenes),
                 batched_scene_steps_per_s=batched, plain_scene_steps_per_s=best,
-                speedup=batched / best, fp64_check_scenes=picks), samples
+                speedup=batched / best, graph_step_us=round(1e6 * t_b / steps, 2),
+                kernel_us_per_step=kern, kernel_sum_us=round(sum(kern.values()), 2),
+                fp64_check_scenes=picks), samples
 
 
 # ----------------------------------------------------- CPU-baseline leg
@@ -294,8 +320,7 @@ def main():
     ms_pair, n_pair = eng.kernel_time(PAIR_KERNEL) if not args.graph else (0.0, 0)
     kernels = {}
     if not args.graph:
-        for k in ("hash", "scan", "scatter", "rank_move", "shepard", "nl_build", "init_tmp", "wc_eos", "wc_bdy",
-                  "wc_fluid", "lf_half", "lf_full", "advect_pos", "chk_density", "finish"):
+        for k in STEP_KERNELS:
             ms, n = eng.kernel_time(k)
             if n:
                 kernels[k] = round(ms * n / args.steps, 5)
