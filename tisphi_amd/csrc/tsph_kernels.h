@@ -221,16 +221,46 @@ __global__ __launch_bounds__(256) void k_scatter(KArgs a) {
 // counting_sort (particle_system.py:238-252) with a deterministic slot:
 // rank among the particles of the same cell by previous index (the serial
 // reference order), then move the persistent state to the other buffer.
+// The slots of a workgroup lie in the cells of its first and last slot;
+// those cells' arrival slots are staged in LDS once, so the rank count of
+// every thread reads LDS (lanes of one cell read the same words) instead
+// of the texture path (the global-memory count kept the TD unit ~94 % busy).
+constexpr int kRankStage = 1024;
 __device__ __forceinline__ float wc_eos(const SceneDev& S, float rt);
 __global__ __launch_bounds__(256) void k_rank_move(KArgs a, int move_vtmp, int move_mui,
                                                    int move_dp, int fuse_init) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.w.start[a.ccap]) return;
-  const int p = a.w.tmp[k];
-  const int32_t key = a.w.key[p];
-  const int b = a.w.start[key], e = a.w.start[key + 1];
+  __shared__ int32_t win[kRankStage];
+  __shared__ int32_t wlo, whi;
+  const int total = a.w.start[a.ccap];
+  const int k0 = blockIdx.x * blockDim.x;
+  if (k0 >= total) return;  // workgroup-uniform
+  const int k = k0 + (int)threadIdx.x;
+  const bool live = k < total;
+  int p = 0, b = 0, e = 0;
+  int32_t key = 0;
+  if (live) {
+    p = a.w.tmp[k];
+    key = a.w.key[p];
+    b = a.w.start[key];
+    e = a.w.start[key + 1];
+  }
+  const int klast = min(k0 + (int)blockDim.x, total) - 1;
+  if (k == k0) wlo = b;
+  if (k == klast) whi = e;
+  __syncthreads();
+  const int lo = wlo, nw = whi - lo;
+  const bool staged = nw <= kRankStage;  // workgroup-uniform
+  if (staged) {
+    for (int q = (int)threadIdx.x; q < nw; q += (int)blockDim.x) win[q] = a.w.tmp[lo + q];
+    __syncthreads();
+  }
+  if (!live) return;
   int rank = 0;
-  for (int q = b; q < e; ++q) rank += (a.w.tmp[q] < p) ? 1 : 0;
+  if (staged) {
+    for (int q = b; q < e; ++q) rank += (win[q - lo] < p) ? 1 : 0;
+  } else {
+    for (int q = b; q < e; ++q) rank += (a.w.tmp[q] < p) ? 1 : 0;
+  }
   const int s = a.w.sid[p];
   const SceneDev S = a.scenes[s];
   const int d = S.pbase + (b - a.w.start[S.cbase]) + rank;
@@ -728,31 +758,37 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   if (!tile_thread(a, s, p)) return;
   if (!is_solid(a.cur.type[p])) return;  // boundary and rigid (solver_sph_wc.py:90-103)
   const SceneDev S = a.scenes[s];
-  const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float* __restrict__ pr = a.w.p;
   const float gy = S.grav[1];
+  // Most dummy particles have no flow particle in reach (an empty list):
+  // their sums stay zero, so they skip the centre and the walk and only
+  // form the extrapolation below (the same values the empty walk gives).
+  const float4 v = a.cur.vel[p];
+  const float cs_in = SHEP ? 1.0f : a.w.cspm[p];
   float svx = 0.f, svy = 0.f, svz = 0.f, sp = 0.f, shep = 0.f;
-  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
-                                                  float mVj, int) {
-    if (!is_flow(type[j])) return;
-    const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
-    if (SHEP) shep += mVj * w;
-    const float4 vj = vtmp[j];
-    svx += mVj * vj.x * w;
-    svy += mVj * vj.y * w;
-    svz += mVj * vj.z * w;
-    sp += mVj * (pr[j] + vj.w * gy * y) * w;
-  });
+  if (!NL || a.w.nlc[p] != 0) {
+    const Centre C = centre_of(a, S, p);
+    neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                    float mVj, int) {
+      if (!is_flow(type[j])) return;
+      const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
+      if (SHEP) shep += mVj * w;
+      const float4 vj = vtmp[j];
+      svx += mVj * vj.x * w;
+      svy += mVj * vj.y * w;
+      svz += mVj * vj.z * w;
+      sp += mVj * (pr[j] + vj.w * gy * y) * w;
+    });
+  }
   float cs;
   if (SHEP) {
     cs = shep != 0.0f ? 1.0f / shep : 1.0f;
     a.w.cspm[p] = cs;
   } else {
-    cs = a.w.cspm[p];
+    cs = cs_in;
   }
-  const float4 v = a.cur.vel[p];
   const float4 vt = make_float4(2.0f * v.x - svx * cs, 2.0f * v.y - svy * cs,
                                 2.0f * v.z - svz * cs, S.rho0);
   a.cur.vtmp[p] = vt;
