@@ -653,7 +653,10 @@ __global__ __launch_bounds__(256) void k_shepard_list(KArgs a, int skip_fluid) {
   float sum = 0.0f;
   neighbours_k<DIM, true>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                   float mVj, int) {
-    if (is_flow(type[j])) sum += mVj * kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
+    // straight-line: a non-flow neighbour adds +0 (the sum's bits stay)
+    const bool fl = is_flow(type[j]);
+    const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
+    sum += fl ? mVj * w : 0.0f;
   });
   a.w.cspm[p] = sum != 0.0f ? 1.0f / sum : 1.0f;
 }
@@ -770,16 +773,21 @@ __global__ __launch_bounds__(256) void k_wc_bdy(KArgs a) {
   float svx = 0.f, svy = 0.f, svz = 0.f, sp = 0.f, shep = 0.f;
   if (!NL || a.w.nlc[p] != 0) {
     const Centre C = centre_of(a, S, p);
+    // Straight-line pair terms (a non-flow neighbour adds +0, which leaves
+    // every sum's bits unchanged: the sums start at +0 and never become -0
+    // in round-to-nearest), so the loads of a batch of neighbours are
+    // issued together instead of one branch at a time.
     neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
                                                     float mVj, int) {
-      if (!is_flow(type[j])) return;
-      const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
-      if (SHEP) shep += mVj * w;
+      const bool fl = is_flow(type[j]);
       const float4 vj = vtmp[j];
-      svx += mVj * vj.x * w;
-      svy += mVj * vj.y * w;
-      svz += mVj * vj.z * w;
-      sp += mVj * (pr[j] + vj.w * gy * y) * w;
+      const float prj = pr[j];
+      const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
+      if (SHEP) shep += fl ? mVj * w : 0.0f;
+      svx += fl ? mVj * vj.x * w : 0.0f;
+      svy += fl ? mVj * vj.y * w : 0.0f;
+      svz += fl ? mVj * vj.z * w : 0.0f;
+      sp += fl ? mVj * (prj + vj.w * gy * y) * w : 0.0f;
     });
   }
   float cs;
@@ -1064,27 +1072,34 @@ __global__ __launch_bounds__(256) void k_soil_bdy(KArgs a) {
   if (!tile_thread(a, s, p)) return;
   if (!is_solid(a.cur.type[p])) return;  // boundary and rigid (muI :96, DP :221)
   const SceneDev S = a.scenes[s];
-  const Centre C = centre_of(a, S, p);
   const int8_t* __restrict__ type = a.cur.type;
   const float4* __restrict__ vtmp = a.cur.vtmp;
   const float* __restrict__ sgt = a.w.sigt_f;
   const int st = a.stride;
   float sv[3] = {0.f, 0.f, 0.f}, sr = 0.f, ssg[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
-                                                  float mVj, int) {
-    if (!is_flow(type[j])) return;
-    const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
-    const float4 vj = vtmp[j];
-    sv[0] += mVj * vj.x * w;
-    sv[1] += mVj * vj.y * w;
-    sv[2] += mVj * vj.z * w;
-    sr += mVj * vj.w * w;
-    const float xr[3] = {x, y, z};
+  // as in k_wc_bdy: an empty list skips the walk (the sums stay +0), and
+  // the pair terms are straight-line (a non-flow neighbour adds +0)
+  if (!NL || a.w.nlc[p] != 0) {
+    const Centre C = centre_of(a, S, p);
+    neighbours_k<DIM, NL>(S, a, p, C, [&](auto kt, int j, float x, float y, float z, float r2,
+                                                    float mVj, int) {
+      const bool fl = is_flow(type[j]);
+      const float w = kernel_W_t<decltype(kt)::v>(S, pair_dist(r2));
+      const float4 vj = vtmp[j];
+      float sj[6];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) ssg[c] += mVj * (sgt[c * st + j] + (vj.w * S.grav[c]) * xr[c]) * w;
+      for (int c = 0; c < 6; ++c) sj[c] = sgt[c * st + j];
+      sv[0] += fl ? mVj * vj.x * w : 0.0f;
+      sv[1] += fl ? mVj * vj.y * w : 0.0f;
+      sv[2] += fl ? mVj * vj.z * w : 0.0f;
+      sr += fl ? mVj * vj.w * w : 0.0f;
+      const float xr[3] = {x, y, z};
 #pragma unroll
-    for (int c = 3; c < 6; ++c) ssg[c] += mVj * sgt[c * st + j] * w;
-  });
+      for (int c = 0; c < 3; ++c) ssg[c] += fl ? mVj * (sj[c] + (vj.w * S.grav[c]) * xr[c]) * w : 0.0f;
+#pragma unroll
+      for (int c = 3; c < 6; ++c) ssg[c] += fl ? mVj * sj[c] * w : 0.0f;
+    });
+  }
   const float cs = a.w.cspm[p];
   const float4 v = a.cur.vel[p];
   float rt = S.rho0;
