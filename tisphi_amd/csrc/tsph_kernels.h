@@ -359,7 +359,12 @@ struct ListAppender {
   // compare picks the slot)
   uint32_t b0 = 0, b1 = 0, b2 = 0;
   __device__ __forceinline__ void add(uint32_t e) {
-    if ((n & 3) == 3 && n < cap) *reinterpret_cast<uint4*>(li + nl_at(n - 3)) = make_uint4(b0, b1, b2, e);
+    // entries n-3..n go to nl_at(n - 3) = (n >> 4) * 1024 + (n & 12): a
+    // 32-bit byte offset (lists stay far below 4 GiB per particle)
+    if ((n & 3) == 3 && n < cap) {
+      const uint32_t off = (((uint32_t)n >> 4) << 12) | (((uint32_t)n << 2) & 48u);
+      *reinterpret_cast<uint4*>(reinterpret_cast<char*>(li) + off) = make_uint4(b0, b1, b2, e);
+    }
     b0 = b1;
     b1 = b2;
     b2 = e;
@@ -609,7 +614,11 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
           // carries the test, and the record is read whole (one 16-byte LDS
           // read) so the flow flag costs no second read on the hit path:
           // 10 vector instructions per rejected candidate instead of 12.
+          // The entry of the staged record at byte offset o (a multiple of
+          // 16) is nl_entry(jlo + o / 16, code) = (o << 2) + kbase: one
+          // shift-add on the hit path.
           const uint32_t oend = (uint32_t)(je - jlo) * 16u;
+          const uint32_t kbase = nl_entry(jlo, code);
           const char* __restrict__ sb = reinterpret_cast<const char*>(sp);
           auto scan = [&](uint32_t skip, auto own) {
 #pragma unroll 1
@@ -621,11 +630,8 @@ __global__ __launch_bounds__(64) void k_nl_build_lds(KArgs a, int use_cflow, int
               const float y = by - pj.y;
               const float z = DIM == 3 ? bz - pj.z : 0.0f;
               const float r2 = fmaf(z, z, fmaf(y, y, x * x));
-              if (r2 < S.rs2 && (!decltype(own)::value || o != skip) && pj.w >= wmin) {
-                uint32_t oo = o;
-                asm volatile("" : "+v"(oo));
-                ap.add(nl_entry(jlo + (int)(oo >> 4), code));
-              }
+              if (r2 < S.rs2 && (!decltype(own)::value || o != skip) && pj.w >= wmin)
+                ap.add((o << 2) + kbase);
             }
           };
           if (dx == 0 && dy == 0 && dz == 0)
