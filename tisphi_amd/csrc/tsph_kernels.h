@@ -14,13 +14,26 @@
 namespace tsph {
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Remap so that every XCD walks one contiguous stretch of tiles:
-// neighbouring particles (and their shared neighbour data) stay in one L2.
+// L2.  Remap so that every XCD walks kXcdStretches contiguous stretches of
+// tiles, stretch c going to XCD c % 8: neighbouring particles (and their
+// shared neighbour data) stay in one L2 within a stretch, and every XCD gets
+// an even share of the work when the work is not uniform along the tile
+// range.  (One stretch per XCD left the 3D dam break's last two XCDs with
+// 4 % of the fluid work: its water sits in the first 70 % of the particle
+// order, and the busiest XCD set the pass time at 1.33x the even share.)
+// The map is a bijection of [0, gridDim.x): block b on XCD x = b % 8 at
+// position i = b / 8 takes logical block ((i / S) * 8 + x) * S + i % S for
+// b below the largest multiple of 8 S, and b itself above it.
 constexpr int kXcds = 8;
+constexpr int kXcdStretches = 8;
 __device__ __forceinline__ int xcd_block() {
   const int nb = gridDim.x, b = blockIdx.x;
-  const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
-  return x * q + (x < r ? x : r) + b / kXcds;
+  const int S = max(1, nb / (kXcds * kXcdStretches));  // blocks per stretch
+  const int nmain = nb - nb % (kXcds * S);
+  if (b >= nmain) return b;
+  const int x = b % kXcds, i = b / kXcds;
+  const int j = i / S, o = i - j * S;
+  return (j * kXcds + x) * S + o;
 }
 __device__ __forceinline__ int wave_id() {
   return __builtin_amdgcn_readfirstlane((int)((xcd_block() * blockDim.x + threadIdx.x) >> 6));
