@@ -14,20 +14,27 @@
 namespace tsph {
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Remap so that every XCD walks kXcdStretches contiguous stretches of
-// tiles, stretch c going to XCD c % 8: neighbouring particles (and their
-// shared neighbour data) stay in one L2 within a stretch, and every XCD gets
-// an even share of the work when the work is not uniform along the tile
-// range.  (One stretch per XCD left the 3D dam break's last two XCDs with
-// 4 % of the fluid work: its water sits in the first 70 % of the particle
-// order, and the busiest XCD set the pass time at 1.33x the even share.)
-// The map is a bijection of [0, gridDim.x): block b on XCD x = b % 8 at
+// L2.  A grid that the chip holds at once (about 2048 threads per CU) is
+// remapped so that every XCD walks one contiguous stretch of tiles:
+// neighbouring particles (and their shared neighbour data) stay in one L2.
+// A larger grid is remapped into kXcdStretches stretches per XCD, stretch c
+// going to XCD c % 8: locality within a stretch, and an even share of the
+// work per XCD when the work is not uniform along the tile range.  (One
+// stretch per XCD left the 3D dam break's last two XCDs with 4 % of the
+// fluid work: its water sits in the first 70 % of the particle order, and
+// the busiest XCD set the pass time at 1.33x the even share.)  Both maps
+// are bijections of [0, gridDim.x).  Stretched: block b on XCD x = b % 8 at
 // position i = b / 8 takes logical block ((i / S) * 8 + x) * S + i % S for
 // b below the largest multiple of 8 S, and b itself above it.
 constexpr int kXcds = 8;
-constexpr int kXcdStretches = 8;
+constexpr int kXcdStretches = 32;  // measured: 4 / 8 / 16 / 32 / 64 -> 4.03 / 3.99 / 3.96 / 3.94 / 3.95 ms per 3D step
+constexpr int kOneRoundThreads = 256 * 2048;
 __device__ __forceinline__ int xcd_block() {
   const int nb = gridDim.x, b = blockIdx.x;
+  if (nb * (int)blockDim.x <= kOneRoundThreads) {
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+  }
   const int S = max(1, nb / (kXcds * kXcdStretches));  // blocks per stretch
   const int nmain = nb - nb % (kXcds * S);
   if (b >= nmain) return b;
